@@ -1,0 +1,127 @@
+"""Benchmark: FP64 SIPG Laplace vmult throughput (GDoF/s) on one MI355X.
+
+Workload (default): affine Cartesian [0,1]^3, 64^3 cells, degree 4 (32,768,000 DoF), FP64,
+homogeneous Dirichlet operator, input seeded uniform[-1,1] (SplitMix64), already in HBM.
+One step = one vmult over the whole vector. Prints ONE JSON line.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DG Laplace vmult GDoF/s; time-to-1e-10 solve, learned+FP32 MG vs plain"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E vendor peak
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_sipg_cartesian_k4_64c.json")
+
+
+def algorithmic_bytes(n_dofs, n_cells, s=8):
+    """src read + dst write + per-cell data (J^{-1}, JxW, tau): 2 s N + 32 N_c bytes."""
+    return 2 * s * n_dofs + 32 * n_cells
+
+
+def cpu_baseline(degree, n_cells_1d, target_s=10.0):
+    """Oracle FP64 vmult (OpenMP, all host threads) on a bounded sample of the same workload."""
+    import oracle
+    op = oracle.LaplaceOperator(oracle.make_mesh(n_cells_1d), degree)
+    x = oracle.random_vector(op.n)
+    op.vmult(x)
+    times = []
+    t_end = time.perf_counter() + target_s
+    while time.perf_counter() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        op.vmult(x)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+    return {"value": round(op.n / t / 1e9, 6), "unit": "GDoF/s", "cores": cores, "kind": "port",
+            "sample": f"oracle vmult, {n_cells_1d}^3 cells k={degree} ({op.n} DoF) FP64, median of "
+                      f"{len(times)} calls"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cells", type=int, default=64)
+    ap.add_argument("--degree", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import exadg_amd
+    torch.cuda.set_device(0)
+    mesh = exadg_amd.mesh_desc(args.cells)
+    op = exadg_amd.LaplaceOperator(mesh, exadg_amd.bc_desc(), args.degree, exadg_amd.FP64)
+    n = op.n
+    # seeded uniform[-1,1] input, generated on the host once and moved to HBM
+    import oracle
+    x = torch.from_numpy(oracle.random_vector(n)).cuda()
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        op.vmult(y, x)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        op.vmult(y, x)
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [s.elapsed_time(e) for s, e in ev]
+    ms_step = wall * 1e3 / args.steps
+    t_med = statistics.median(kernel_ms) * 1e-3
+    ab = algorithmic_bytes(n, op.n_cells)
+    achieved = ab / t_med / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        try:
+            traffic = json.load(open(PROFILE_TRAFFIC)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": round(n / (ms_step * 1e-3) / 1e9, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"SIPG vmult, affine Cartesian [0,1]^3, {args.cells}^3 cells, "
+                               f"degree {args.degree}, FP64, homogeneous Dirichlet",
+                   "n_dofs": n, "n_cells": op.n_cells, "degree": args.degree,
+                   "kernel": "sipg_cartesian_kernel",
+                   "kernel_ms_median": round(statistics.median(kernel_ms), 5),
+                   "kernel_ms_min": round(min(kernel_ms), 5)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.degree, 32)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

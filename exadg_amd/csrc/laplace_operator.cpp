@@ -1,0 +1,149 @@
+#include "laplace_operator.h"
+
+#include <cmath>
+
+namespace ExaDG
+{
+namespace Poisson
+{
+template<typename Number>
+LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> grid,
+                                         LaplaceOperatorData const &           data)
+  : grid_(std::move(grid)), data_(data)
+{
+  if(data.degree < 1 || data.degree > 8)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "degree must be in [1,8]");
+  shape_     = make_shape_info(data.degree);
+  int const n = shape_.n;
+  n3_         = n * n * n;
+  bool const curved = !grid_->affine();
+  geo_              = compute_geometry(*grid_, shape_, data.IP_factor, curved);
+
+  std::vector<Number> tab;
+  if(curved)
+  {
+    for(double v : shape_.S)
+      tab.push_back((Number)v);
+    for(double v : shape_.Dq)
+      tab.push_back((Number)v);
+    for(double v : shape_.d0)
+      tab.push_back((Number)v);
+    for(double v : shape_.d1)
+      tab.push_back((Number)v);
+  }
+  else
+  {
+    for(int d = 0; d < 3; ++d)
+    {
+      double const h = grid_->h(d);
+      for(double v : shape_.mass)
+        tab.push_back((Number)(h * v));
+      for(double v : shape_.stiff)
+        tab.push_back((Number)(v / h));
+      for(double v : shape_.d0)
+        tab.push_back((Number)(v / h));
+      for(double v : shape_.d1)
+        tab.push_back((Number)(v / h));
+    }
+  }
+  tables_.upload(tab);
+  auto convert = [](std::vector<double> const & v) {
+    return std::vector<Number>(v.begin(), v.end());
+  };
+  tau_face_.upload(convert(geo_.tau_face));
+  if(curved)
+  {
+    cell_metric_.upload(convert(geo_.cell_metric));
+    face_data_.upload(convert(geo_.face_data));
+    // the metric lives on the device; keep only tau on the host
+    geo_.cell_metric.clear();
+    geo_.cell_metric.shrink_to_fit();
+    geo_.face_data.clear();
+    geo_.face_data.shrink_to_fit();
+  }
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::apply(ApplyMode mode, Number * dst, Number const * src, int probe,
+                                    hipStream_t s) const
+{
+  if(mode != ApplyMode::diagonal && (dst == nullptr || src == nullptr))
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  if(mode != ApplyMode::diagonal && (void const *)dst == (void const *)src)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "dst and src must not alias");
+  SipgArgs<Number> a;
+  a.src         = src;
+  a.dst         = dst;
+  a.tables      = tables_.data();
+  a.cell_metric = cell_metric_.data();
+  a.face_data   = face_data_.data();
+  a.tau_face    = tau_face_.data();
+  a.topo        = grid_->topology();
+  a.n_cells     = grid_->n_cells();
+  a.probe       = probe;
+  hipError_t const e = grid_->affine() ? launch_sipg_cartesian<Number>(data_.degree, mode, a, s)
+                                       : launch_sipg_general<Number>(data_.degree, mode, a, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, std::string("SIPG kernel launch: ") + hipGetErrorString(e));
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::vmult(Number * dst, Number const * src, hipStream_t s) const
+{
+  apply(ApplyMode::vmult, dst, src, 0, s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::vmult_add(Number * dst, Number const * src, hipStream_t s) const
+{
+  apply(ApplyMode::vmult_add, dst, src, 0, s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::calculate_diagonal(Number * diag, hipStream_t s) const
+{
+  if(diag == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  // one launch per local unit vector; each launch writes entry `probe` of every cell
+  for(int probe = 0; probe < n3_; ++probe)
+    apply(ApplyMode::diagonal, diag, nullptr, probe, s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::calculate_inverse_diagonal(Number * diag, hipStream_t s) const
+{
+  calculate_diagonal(diag, s);
+  invert_diagonal_device<Number>(diag, m(), s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::rhs_sine(double omega, double * b, hipStream_t s) const
+{
+  if(b == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  std::vector<double> hb;
+  assemble_rhs_sine(*grid_, shape_, geo_, omega, hb);
+  HIP_CHECK(hipMemcpyAsync(b, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+std::unique_ptr<LaplaceOperatorBase> create_laplace_operator(exadg_mesh_desc const & mesh,
+                                                             exadg_bc_desc const & bc, int degree,
+                                                             int precision, double ip_factor)
+{
+  auto                grid = std::make_shared<StructuredGrid const>(mesh, bc);
+  LaplaceOperatorData data;
+  data.degree    = degree;
+  data.IP_factor = ip_factor;
+  if(precision == EXADG_FP64)
+    return std::make_unique<LaplaceOperator<double>>(grid, data);
+  if(precision == EXADG_FP32)
+    return std::make_unique<LaplaceOperator<float>>(grid, data);
+  EXADG_THROW(EXADG_ERR_ARGUMENT, "precision must be EXADG_FP64 or EXADG_FP32");
+}
+
+template class LaplaceOperator<double>;
+template class LaplaceOperator<float>;
+
+}  // namespace Poisson
+}  // namespace ExaDG

@@ -1,0 +1,108 @@
+// Host-side SIPG Laplace operator: the MI355X counterpart of ExaDG's
+// Poisson::LaplaceOperator<dim=3, Number, 1> on top of OperatorBase
+// (poisson/spatial_discretization/laplace_operator.h:227-351, operators/operator_base.h:105-756).
+// Vectors are device pointers of length n(); everything is stream ordered.
+#pragma once
+#include <memory>
+
+#include "common.h"
+#include "grid.h"
+#include "laplace_kernels.h"
+#include "shape_info.h"
+
+namespace ExaDG
+{
+namespace Poisson
+{
+// LaplaceOperatorData / LaplaceKernelData (laplace_operator.h:38-45,212-225)
+struct LaplaceOperatorData
+{
+  int    degree    = 1;
+  double IP_factor = 1.0;
+};
+
+class LaplaceOperatorBase
+{
+public:
+  virtual ~LaplaceOperatorBase() = default;
+  virtual int     precision() const                                         = 0;
+  virtual int     degree() const                                            = 0;
+  virtual int64_t m() const                                                 = 0;
+  virtual int64_t n_cells() const                                           = 0;
+  virtual bool    is_affine() const                                         = 0;
+  virtual void    vmult(void * dst, void const * src, hipStream_t s) const     = 0;
+  virtual void    vmult_add(void * dst, void const * src, hipStream_t s) const = 0;
+  virtual void    calculate_diagonal(void * diag, hipStream_t s) const         = 0;
+  virtual void    calculate_inverse_diagonal(void * diag, hipStream_t s) const = 0;
+  virtual void    rhs_sine(double omega, double * b, hipStream_t s) const      = 0;
+};
+
+template<typename Number>
+class LaplaceOperator : public LaplaceOperatorBase
+{
+public:
+  LaplaceOperator(std::shared_ptr<StructuredGrid const> grid, LaplaceOperatorData const & data);
+
+  int     precision() const override { return sizeof(Number) == 8 ? EXADG_FP64 : EXADG_FP32; }
+  int     degree() const override { return data_.degree; }
+  int64_t m() const override { return grid_->n_cells() * n3_; }
+  int64_t n() const { return m(); }
+  int64_t n_cells() const override { return grid_->n_cells(); }
+  bool    is_affine() const override { return grid_->affine(); }
+
+  // OperatorBase::vmult / vmult_add / apply (operator_base.cpp:156-178, 264-310)
+  void vmult(Number * dst, Number const * src, hipStream_t s) const;
+  void vmult_add(Number * dst, Number const * src, hipStream_t s) const;
+  // OperatorBase::calculate_diagonal / calculate_inverse_diagonal (operator_base.cpp:249-262,
+  // 608-646): exact diagonal by unit-vector probing of the cell-local block
+  void calculate_diagonal(Number * diag, hipStream_t s) const;
+  void calculate_inverse_diagonal(Number * diag, hipStream_t s) const;
+
+  void vmult(void * dst, void const * src, hipStream_t s) const override
+  {
+    vmult(static_cast<Number *>(dst), static_cast<Number const *>(src), s);
+  }
+  void vmult_add(void * dst, void const * src, hipStream_t s) const override
+  {
+    vmult_add(static_cast<Number *>(dst), static_cast<Number const *>(src), s);
+  }
+  void calculate_diagonal(void * d, hipStream_t s) const override
+  {
+    calculate_diagonal(static_cast<Number *>(d), s);
+  }
+  void calculate_inverse_diagonal(void * d, hipStream_t s) const override
+  {
+    calculate_inverse_diagonal(static_cast<Number *>(d), s);
+  }
+  // Poisson::Operator::rhs for the manufactured sine solution (FP64 output)
+  void rhs_sine(double omega, double * b, hipStream_t s) const override;
+
+  StructuredGrid const & grid() const { return *grid_; }
+  ShapeInfo1D const &    shape() const { return shape_; }
+  GeometryData const &   geometry_host() const { return geo_; }
+
+private:
+  void apply(ApplyMode mode, Number * dst, Number const * src, int probe, hipStream_t s) const;
+
+  std::shared_ptr<StructuredGrid const> grid_;
+  LaplaceOperatorData                   data_;
+  ShapeInfo1D                           shape_;
+  GeometryData                          geo_;  // host copy (tau, and metric for curved)
+  int                                   n3_;
+  DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
+};
+
+std::unique_ptr<LaplaceOperatorBase> create_laplace_operator(exadg_mesh_desc const & mesh,
+                                                             exadg_bc_desc const & bc, int degree,
+                                                             int precision, double ip_factor);
+
+// b = RHS of the manufactured solution prod sin(omega x_d) (host, FP64)
+void assemble_rhs_sine(StructuredGrid const & grid, ShapeInfo1D const & shape,
+                       GeometryData const & geo, double omega, std::vector<double> & b);
+
+// dst[i] = 1/d[i] with |d| <= 1e-10 -> 1 (utilities/invert_diagonal.h:35-46)
+template<typename Number>
+void invert_diagonal_device(Number * d, int64_t n, hipStream_t s);
+
+}  // namespace Poisson
+}  // namespace ExaDG

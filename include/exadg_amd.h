@@ -1,0 +1,114 @@
+/*
+ * exadg_amd.h -- C ABI of the MI355X-native SIPG Laplace hot path (ExaDG drop-in).
+ *
+ * Every entry point returns an int status (EXADG_OK = 0) and never throws. Vectors are
+ * DEVICE pointers (HBM) of length n_dofs in the precision the handle was created with,
+ * unless stated otherwise. Calls are ordered on the HIP stream passed as `stream`
+ * (a hipStream_t; NULL = the null stream). Handles are not thread-safe: one handle must
+ * not be used by two host threads at once (the reference operators mutate `mutable`
+ * state in const methods, operator_base.h:473, laplace_operator.h:206).
+ *
+ * DoF layout: cells lexicographic over the structured box (x fastest), then the
+ * (k+1)^3 nodal values of FE_DGQ(k) on Gauss-Lobatto points, x fastest.
+ */
+#ifndef EXADG_AMD_H
+#define EXADG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+enum exadg_status
+{
+  EXADG_OK               = 0,
+  EXADG_ERR_ARGUMENT     = 1, /* bad argument (sizes, aliasing, unsupported degree, ...)   */
+  EXADG_ERR_HIP          = 2, /* HIP runtime error                                          */
+  EXADG_ERR_NOT_FINITE   = 3, /* NaN/Inf detected (AssertThrow(std::isfinite) in the ref.) */
+  EXADG_ERR_NO_CONVERGE  = 4, /* iteration limit reached (SolverControl::NoConvergence)     */
+  EXADG_ERR_NOT_READY    = 5, /* object needs update() before use                           */
+  EXADG_ERR_INTERNAL     = 6
+};
+
+enum exadg_precision
+{
+  EXADG_FP64 = 0,
+  EXADG_FP32 = 1
+};
+
+enum exadg_boundary_type
+{
+  EXADG_BC_DIRICHLET = 0,
+  EXADG_BC_NEUMANN   = 1
+};
+
+/* Message of the last error on the calling thread (never NULL). */
+const char * exadg_last_error(void);
+
+/* ---------------------------------------------------------------- mesh / operator */
+
+/* Structured hexahedral box, the grids the reference builds with
+ * GridGenerator::subdivided_hyper_cube (sine/application.h:206) and create_periodic_box
+ * (grid/periodic_box.h:35-87), optionally deformed by DeformedCubeManifold
+ * (grid/deformed_cube_manifold.h:47-157) and described by MappingQ(mapping_degree)
+ * (grid/grid_utilities.h:52-58). deformation == 0 gives an affine Cartesian mesh. */
+typedef struct exadg_mesh_desc
+{
+  int32_t n_cells[3];
+  double  lower[3];
+  double  upper[3];
+  double  deformation;
+  int32_t frequency;
+  int32_t mapping_degree;
+  int32_t periodic[3];
+} exadg_mesh_desc;
+
+/* Boundary descriptor (poisson/user_interface/boundary_descriptor.h:66-80): the type of
+ * each box side 0:x-, 1:x+, 2:y-, 3:y+, 4:z-, 5:z+ (ignored for periodic directions). */
+typedef struct exadg_bc_desc
+{
+  int32_t type[6];
+} exadg_bc_desc;
+
+typedef struct exadg_laplace exadg_laplace;
+
+/* Replaces LaplaceOperator::initialize (laplace_operator.h:250-254) together with the
+ * MatrixFree / mapping setup of Poisson::Operator::setup (operator.cpp:261-284) and the
+ * penalty computation IP::calculate_penalty_parameter (interior_penalty_parameter.h:43-99).
+ * degree 1..8; precision EXADG_FP64 or EXADG_FP32; ip_factor = LaplaceKernelData::IP_factor. */
+int exadg_laplace_create(const exadg_mesh_desc * mesh, const exadg_bc_desc * bc, int degree,
+                         int precision, double ip_factor, exadg_laplace ** out);
+int exadg_laplace_destroy(exadg_laplace * op);
+
+/* OperatorBase::m() / n() (operator_base.h:187-191). */
+int exadg_laplace_n_dofs(const exadg_laplace * op, int64_t * n_dofs);
+/* Number of cells, degree, precision, and whether the affine (Cartesian) kernel is used. */
+int exadg_laplace_info(const exadg_laplace * op, int64_t * n_cells, int * degree, int * precision,
+                       int * is_affine);
+
+/* OperatorBase::vmult(dst, src) (operator_base.h:175, operator_base.cpp:156-168 ->
+ * apply :264-310): dst = A src for the homogeneous SIPG operator. dst is overwritten;
+ * dst == src is rejected with EXADG_ERR_ARGUMENT. */
+int exadg_laplace_vmult(exadg_laplace * op, void * dst, const void * src, void * stream);
+/* OperatorBase::vmult_add (operator_base.h:178): dst += A src. */
+int exadg_laplace_vmult_add(exadg_laplace * op, void * dst, const void * src, void * stream);
+
+/* OperatorBase::calculate_inverse_diagonal (operator_base.cpp:249-262): diag(A) inverted
+ * with invert_diagonal's guard |d| <= 1e-10 -> 1 (invert_diagonal.h:35-46). */
+int exadg_laplace_calculate_inverse_diagonal(exadg_laplace * op, void * inv_diag, void * stream);
+/* diag(A) without inversion (OperatorBase::calculate_diagonal, operator_base.cpp:608-616). */
+int exadg_laplace_calculate_diagonal(exadg_laplace * op, void * diag, void * stream);
+
+/* Poisson::Operator::rhs (operator.cpp:414-423) for the manufactured solution
+ * u = prod_d sin(omega x_d): f = 3 omega^2 u, Dirichlet g = u, Neumann h = grad(u).n.
+ * b is always FP64. (sine/application.h:32-90 with omega = 3 pi.) */
+int exadg_laplace_rhs_sine(exadg_laplace * op, double omega, double * b, void * stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXADG_AMD_H */

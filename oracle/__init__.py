@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU-baseline leg of
+bench.py as the checker. The product package never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+DIRICHLET, NEUMANN = 0, 1
+
+
+class MeshDesc(ctypes.Structure):
+    """Structured box, optionally deformed (DeformedCubeManifold) with MappingQ geometry."""
+
+    _fields_ = [
+        ("n_cells", ctypes.c_int * 3),
+        ("lower", ctypes.c_double * 3),
+        ("upper", ctypes.c_double * 3),
+        ("deformation", ctypes.c_double),
+        ("frequency", ctypes.c_int),
+        ("mapping_degree", ctypes.c_int),
+        ("periodic", ctypes.c_int * 3),
+        ("boundary_type", ctypes.c_int * 6),
+    ]
+
+
+def make_mesh(n_cells, lower=0.0, upper=1.0, deformation=0.0, frequency=1, mapping_degree=1,
+              periodic=(0, 0, 0), boundary_type=(DIRICHLET,) * 6):
+    if isinstance(n_cells, int):
+        n_cells = (n_cells,) * 3
+    lo = (lower,) * 3 if np.isscalar(lower) else tuple(lower)
+    hi = (upper,) * 3 if np.isscalar(upper) else tuple(upper)
+    m = MeshDesc()
+    m.n_cells[:] = list(n_cells)
+    m.lower[:] = list(lo)
+    m.upper[:] = list(hi)
+    m.deformation = deformation
+    m.frequency = frequency
+    m.mapping_degree = mapping_degree
+    m.periodic[:] = list(periodic)
+    m.boundary_type[:] = list(boundary_type)
+    return m
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_laplace_create.restype = ctypes.c_void_p
+        L.oracle_laplace_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.c_double]
+        L.oracle_laplace_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_laplace_n_dofs.restype = ctypes.c_longlong
+        L.oracle_laplace_n_dofs.argtypes = [ctypes.c_void_p]
+        L.oracle_laplace_vmult.argtypes = [ctypes.c_void_p, dp, dp]
+        L.oracle_laplace_rhs_sine.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        L.oracle_laplace_error_sine.restype = ctypes.c_double
+        L.oracle_laplace_error_sine.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        L.oracle_laplace_diagonal.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_laplace_tau.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_laplace_cell_geometry.argtypes = [ctypes.c_void_p, ctypes.c_longlong, dp, dp]
+        L.oracle_laplace_face_geometry.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
+                                                   ctypes.c_int, dp, dp, dp, dp]
+        L.oracle_tables.argtypes = [ctypes.c_int] + [dp] * 7
+        L.oracle_random_vector.argtypes = [ctypes.c_longlong, ctypes.c_ulonglong, dp]
+        L.oracle_cg_solve.restype = ctypes.c_int
+        L.oracle_cg_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int, dp]
+        L.oracle_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+DEFAULT_SEED = 0x5EED2026
+
+
+def random_vector(n, seed=DEFAULT_SEED):
+    """SplitMix64(seed, index) -> uniform[-1,1], identical on CPU and GPU."""
+    v = np.empty(n)
+    lib().oracle_random_vector(n, seed, _p(v))
+    return v
+
+
+def tables(degree):
+    n = degree + 1
+    out = [np.empty(n), np.empty(n), np.empty(n), np.empty((n, n)), np.empty((n, n)), np.empty(n),
+           np.empty(n)]
+    lib().oracle_tables(degree, *[_p(a) for a in out])
+    return dict(zip(["gl", "gp", "gw", "S", "Sd", "d0", "d1"], out))
+
+
+class LaplaceOperator:
+    """Oracle SIPG Laplace operator (homogeneous), mirrors Poisson::LaplaceOperator."""
+
+    def __init__(self, mesh, degree, ip_factor=1.0):
+        self.mesh = mesh
+        self.degree = degree
+        self._h = lib().oracle_laplace_create(ctypes.byref(mesh), degree, ip_factor)
+        if not self._h:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        self.n = lib().oracle_laplace_n_dofs(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_laplace_destroy(self._h)
+            self._h = None
+
+    def vmult(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.empty(self.n)
+        lib().oracle_laplace_vmult(self._h, _p(dst), _p(src))
+        return dst
+
+    def rhs_sine(self, omega):
+        b = np.empty(self.n)
+        lib().oracle_laplace_rhs_sine(self._h, omega, _p(b))
+        return b
+
+    def error_sine(self, omega, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().oracle_laplace_error_sine(self._h, omega, _p(u))
+
+    def diagonal(self):
+        d = np.empty(self.n)
+        lib().oracle_laplace_diagonal(self._h, _p(d))
+        return d
+
+    def tau(self):
+        nc = self.n // (self.degree + 1) ** 3
+        t = np.empty(nc)
+        lib().oracle_laplace_tau(self._h, _p(t))
+        return t
+
+    def cell_geometry(self, c):
+        n3 = (self.degree + 1) ** 3
+        jinv = np.empty((n3, 3, 3))
+        jxw = np.empty(n3)
+        lib().oracle_laplace_cell_geometry(self._h, c, _p(jinv), _p(jxw))
+        return jinv, jxw
+
+    def face_geometry(self, c, direction, side):
+        n2 = (self.degree + 1) ** 2
+        jinv, jxw, nrm, xq = np.empty((n2, 3, 3)), np.empty(n2), np.empty((n2, 3)), np.empty((n2, 3))
+        lib().oracle_laplace_face_geometry(self._h, c, direction, side, _p(jinv), _p(jxw), _p(nrm),
+                                           _p(xq))
+        return jinv, jxw, nrm, xq
+
+    def cg_solve(self, b, x0=None, rel_tol=1e-10, abs_tol=1e-20, max_iter=100000, jacobi=True):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        info = np.zeros(4)
+        rc = lib().oracle_cg_solve(self._h, int(jacobi), _p(b), _p(x), rel_tol, abs_tol, max_iter,
+                                   _p(info))
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return x, dict(iterations=int(info[0]), r0=info[1], r_final=info[2],
+                       converged=bool(info[3]))
