@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--degree", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run plain device copies of the same vector (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); use --gpus 1",
@@ -85,6 +87,10 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = [s.elapsed_time(e) for s, e in ev]
+    if args.copy_probe:
+        for _ in range(args.steps):
+            y.copy_(x)
+        torch.cuda.synchronize()
     ms_step = wall * 1e3 / args.steps
     t_med = statistics.median(kernel_ms) * 1e-3
     ab = algorithmic_bytes(n, op.n_cells)

@@ -28,7 +28,8 @@ struct SipgArgs
   Number const * tau_face;
   BoxTopology    topo;
   int64_t        n_cells;
-  int            probe;  // diagonal mode: local index of the unit vector
+  int            probe;      // diagonal mode: local index of the unit vector
+  int            tile_rows;  // traversal: y-rows per tile (one tile per XCD)
 };
 
 // degree 1..8; mode as above. Returns hipError of the launch.

@@ -352,8 +352,9 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
   Number *       smem = reinterpret_cast<Number *>(smem_raw);
   int const      slot = threadIdx.x / N2;
   int const      lt   = threadIdx.x - slot * N2;
-  int64_t const  cell = (int64_t)blockIdx.x * CPB + slot;
-  bool const     active = slot < CPB && cell < a.n_cells;
+  int64_t const  pos  = xcd_logical_block(blockIdx.x, gridDim.x) * CPB + slot;
+  bool const     active = slot < CPB && pos < a.n_cells;
+  int64_t const  cell   = active ? traversal_cell(a.topo, pos, a.tile_rows) : 0;
   Number *       A      = smem + (active ? slot : 0) * (N3 + WS);
   Number *       W      = A + N3;
   Number *       Gx = W, *Gy = W + N3, *Gz = W + 2 * N3;
@@ -506,78 +507,83 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
 // On an affine axis-parallel mesh the SIPG operator is exactly a sum of Kronecker products:
 //   A = L_x (x) M_y (x) M_z + M_x (x) L_y (x) M_z + M_x (x) M_y (x) L_z
 // with M_d = h_d * M (1D mass, exact with QGauss(k+1)) and L_d the 1D DG Laplacian along d:
-// K/h_d plus the consistency/symmetry/penalty face terms of both ends, which couple a line
-// to the end value and end derivative of the neighbour's line. Evaluated as
-//   out = M_z (M_y a_x + M_x a_y) + M_y M_x a_z,  a_d = L_d(u, neighbours).
+// K/h_d plus, at each line end, the flux terms of laplace_operator.h:180-197 written with
+// the exterior trace (value w and d/dx_d derivative dw at the shared face):
+//   right end: o_i -= 1/2 d1_i [u], o_{n-1} += -1/2 (du + dw) + tau [u],  [u] = u_{n-1} - w
+//   left end : o_i += 1/2 d0_i [u], o_0     +=  1/2 (du + dw) + tau [u],  [u] = u_0 - w
+// For boundary faces (w, dw) come from the mirror principle (weak_boundary_conditions.h):
+// Dirichlet w = -u, dw = du; Neumann w = u, dw = -du -- one branch-free code path.
+// Evaluated as out = M_z (M_y a_x + M_x a_y) + M_y M_x a_z with a_d = L_d(u, traces).
+// The neighbour lines are read once, at kernel start, and reduced to their traces.
 
-template<int N, typename Number, ApplyMode mode>
+// Padded LDS strides (x, y, z, cell) per n, chosen to spread the 8-byte line accesses of
+// the three sweep orientations over the LDS banks.
+template<int N>
+struct LdsLayout
+{
+  static constexpr int PX = (N == 8) ? 9 : N;
+  static constexpr int PY = (N == 4) ? 19 : (N == 6) ? 39 : (N == 7) ? 52 : (N == 8) ? 72 : N * N;
+  static constexpr int PC = (N == 4) ? 80 : (N == 6) ? 260 : (N == 7) ? 369 : (N == 8) ? 576 : N * N * N;
+};
+
+template<int N, typename Number>
 __device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, Number const (&u)[N],
                                                Number (&o)[N], int const (&type)[2],
-                                               Number const (&tau)[2], Number const * wl,
-                                               Number const * wr, int stride, bool self_nb)
+                                               Number const (&tau)[2], Number const (&wnb)[2],
+                                               Number const (&dwnb)[2])
 {
   Number const * K  = tab + N * N;
   Number const * d0 = tab + 2 * N * N;
   Number const * d1 = d0 + N;
   mat_apply<N>(K, u, o);
-  if(type[0] != EXADG_BC_NEUMANN)
-  {  // left end, outward normal -e_d
-    bool const   dir  = type[0] == EXADG_BC_DIRICHLET;
-    Number const f    = dir ? Number(1) : Number(0.5);
-    Number const pen  = dir ? Number(2) * tau[0] : tau[0];
-    Number const du0  = dot_line<N>(d0, u);
-    Number const u0   = u[0];
+  Number const du0 = dot_line<N>(d0, u), du1 = dot_line<N>(d1, u);
+  // exterior traces: neighbour values or mirror values
+  Number const sl  = type[0] == EXADG_BC_DIRICHLET ? Number(-1) : Number(1);
+  Number const sr  = type[1] == EXADG_BC_DIRICHLET ? Number(-1) : Number(1);
+  Number const wl  = type[0] == kInterior ? wnb[0] : sl * u[0];
+  Number const dwl = type[0] == kInterior ? dwnb[0] : -sl * du0;
+  Number const wr  = type[1] == kInterior ? wnb[1] : sr * u[N - 1];
+  Number const dwr = type[1] == kInterior ? dwnb[1] : -sr * du1;
+  Number const jl = Number(0.5) * (u[0] - wl), jr = Number(0.5) * (u[N - 1] - wr);
 #pragma unroll
-    for(int i = 0; i < N; ++i)
-      o[i] += f * d0[i] * u0;
-    o[0] += f * du0 + pen * u0;
-    if(type[0] == kInterior && (mode != ApplyMode::diagonal || self_nb))
+  for(int i = 0; i < N; ++i)
+    o[i] += d0[i] * jl - d1[i] * jr;
+  o[0] += Number(0.5) * (du0 + dwl) + Number(2) * tau[0] * jl;
+  o[N - 1] += Number(2) * tau[1] * jr - Number(0.5) * (du1 + dwr);
+}
+
+// Coordinates of the cell at traversal position pos0 + slot; the divisions act on the
+// block-uniform pos0 only, per lane there is at most a short carry along x.
+__device__ __forceinline__ void traversal_coords(BoxTopology const & t, int tile_rows, int pos0,
+                                                 int slot, int (&cc)[3])
+{
+  int const nx = t.nc[0], ny = t.nc[1], nz = t.nc[2];
+  int const full = nx * nz * tile_rows;
+  int       tile = pos0 / full;
+  int const r    = pos0 - tile * full;
+  int       rows = min(tile_rows, ny - tile * tile_rows);
+  int       x    = r % nx;
+  int const r2   = r / nx;
+  int       yi   = r2 % rows;
+  int       z    = r2 / rows;
+  x += slot;
+  while(x >= nx)
+  {
+    x -= nx;
+    if(++yi == rows)
+    {
+      yi = 0;
+      if(++z == nz)
       {
-        Number w[N];
-        if constexpr(mode == ApplyMode::diagonal)
-        {  // one cell along a periodic direction: the cell is its own neighbour
-#pragma unroll
-          for(int i = 0; i < N; ++i)
-            w[i] = u[i];
-        }
-        else
-          load_line<N>(wl, stride, w);
-        Number const wn = w[N - 1], dw = dot_line<N>(d1, w);
-#pragma unroll
-        for(int i = 0; i < N; ++i)
-          o[i] -= Number(0.5) * d0[i] * wn;
-        o[0] += Number(0.5) * dw - tau[0] * wn;
+        z = 0;
+        ++tile;
+        rows = min(tile_rows, ny - tile * tile_rows);
       }
+    }
   }
-  if(type[1] != EXADG_BC_NEUMANN)
-  {  // right end, outward normal +e_d
-    bool const   dir = type[1] == EXADG_BC_DIRICHLET;
-    Number const f   = dir ? Number(1) : Number(0.5);
-    Number const pen = dir ? Number(2) * tau[1] : tau[1];
-    Number const du1 = dot_line<N>(d1, u);
-    Number const un  = u[N - 1];
-#pragma unroll
-    for(int i = 0; i < N; ++i)
-      o[i] -= f * d1[i] * un;
-    o[N - 1] += pen * un - f * du1;
-    if(type[1] == kInterior && (mode != ApplyMode::diagonal || self_nb))
-      {
-        Number w[N];
-        if constexpr(mode == ApplyMode::diagonal)
-        {
-#pragma unroll
-          for(int i = 0; i < N; ++i)
-            w[i] = u[i];
-        }
-        else
-          load_line<N>(wr, stride, w);
-        Number const w0 = w[0], dw = dot_line<N>(d0, w);
-#pragma unroll
-        for(int i = 0; i < N; ++i)
-          o[i] += Number(0.5) * d1[i] * w0;
-        o[N - 1] -= Number(0.5) * dw + tau[1] * w0;
-      }
-  }
+  cc[0] = x;
+  cc[1] = tile * tile_rows + yi;
+  cc[2] = z;
 }
 
 template<int N, typename Number, ApplyMode mode>
@@ -585,143 +591,224 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
 {
   constexpr int N2 = N * N, N3 = N * N * N, TB = 2 * N * N + 2 * N;
   constexpr int CPB = cells_per_block(N);
+  using L           = LdsLayout<N>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Number *      smem   = reinterpret_cast<Number *>(smem_raw);
   int const     slot   = threadIdx.x / N2;
   int const     lt     = threadIdx.x - slot * N2;
-  int64_t const cell   = (int64_t)blockIdx.x * CPB + slot;
-  bool const    active = slot < CPB && cell < a.n_cells;
-  Number *      U      = smem + (active ? slot : 0) * 3 * N3;
-  Number *      Abuf   = U + N3;
-  Number *      Bbuf   = Abuf + N3;
+  int const     pos0   = (int)xcd_logical_block(blockIdx.x, gridDim.x) * CPB;
+  bool const    active = slot < CPB && pos0 + slot < (int)a.n_cells;
+  Number *      U      = smem + (active ? slot : 0) * 3 * L::PC;
+  Number *      Abuf   = U + L::PC;
+  Number *      Bbuf   = Abuf + L::PC;
   int const     p = lt % N, q = lt / N;
+  // LDS line bases of this lane for the x, y and z sweeps
+  int const     bx = L::PX * p + L::PY * q, by = p + L::PY * q, bz = p + L::PX * q;
 
-  int     cc[3] = {0, 0, 0};
-  int64_t nb[3][2];
-  int     type[3][2];
-  Number  tau[3][2];
+  int    type[3][2];
+  Number tau[3][2], wnb[3][2], dwnb[3][2];
+  int    cell       = 0;
+  bool   in_block_l = false, in_block_r = false;
   if(active)
   {
-    cell_coords(a.topo, cell, cc);
+    int cc[3];
+    traversal_coords(a.topo, a.tile_rows, pos0, slot, cc);
+    int const nx = a.topo.nc[0], ny = a.topo.nc[1];
+    int const stride_c[3] = {1, nx, nx * ny};
+    cell                  = cc[0] + nx * (cc[1] + ny * cc[2]);
+    int nb[3][2];
 #pragma unroll
     for(int d = 0; d < 3; ++d)
+    {
+      int const n_d = a.topo.nc[d], per = a.topo.periodic[d];
+      int const t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+      int const fbase =
+        (int)a.topo.face_base[d] + a.topo.layers[d] * (cc[t1] + a.topo.nc[t1] * cc[t2]);
+      nb[d][0] = cc[d] > 0 ? cell - stride_c[d] : (per ? cell + (n_d - 1) * stride_c[d] : -1);
+      nb[d][1] = cc[d] < n_d - 1 ? cell + stride_c[d] : (per ? cell - (n_d - 1) * stride_c[d] : -1);
+      int const fr = (per && cc[d] + 1 == n_d) ? 0 : cc[d] + 1;
+      tau[d][0]    = a.tau_face[fbase + cc[d]];
+      tau[d][1]    = a.tau_face[fbase + fr];
 #pragma unroll
       for(int s = 0; s < 2; ++s)
-      {
-        nb[d][s]   = neighbor_id(a.topo, cc, d, s);
         type[d][s] = nb[d][s] >= 0 ? kInterior : a.topo.bc[2 * d + s];
-        tau[d][s]  = a.tau_face[face_id(a.topo, cc, d, s)];
-      }
+    }
     if constexpr(mode == ApplyMode::diagonal)
-      for(int e = lt; e < N3; e += N2)
-        U[e] = (e == a.probe) ? Number(1) : Number(0);
+    {
+#pragma unroll
+      for(int m = 0; m < N; ++m)
+        U[bz + L::PY * m] = (lt + m * N2 == a.probe) ? Number(1) : Number(0);
+      // neighbours contribute zero, except a cell that is its own periodic neighbour
+#pragma unroll
+      for(int d = 0; d < 3; ++d)
+      {
+        Number const * tb = a.tables + d * TB;
+        int const      gbase = d == 0 ? N * (p + N * q) : d == 1 ? p + N2 * q : p + N * q;
+        int const      gstr  = d == 0 ? 1 : d == 1 ? N : N2;
+        Number         w[N];
+#pragma unroll
+        for(int i = 0; i < N; ++i)
+          w[i] = (gbase + i * gstr == a.probe) ? Number(1) : Number(0);
+        bool const self = nb[d][0] == cell;
+        wnb[d][0]       = self ? w[N - 1] : Number(0);
+        dwnb[d][0]      = self ? dot_line<N>(tb + 2 * N2 + N, w) : Number(0);
+        wnb[d][1]       = self ? w[0] : Number(0);
+        dwnb[d][1]      = self ? dot_line<N>(tb + 2 * N2, w) : Number(0);
+      }
+    }
     else
-      for(int e = lt; e < N3; e += N2)
-        U[e] = a.src[cell * N3 + e];
+    {
+      // all global loads up front: own cell and the six neighbour lines of this lane
+      Number const * own_p = a.src + (size_t)cell * N3 + lt;
+      Number         own[N];
+#pragma unroll
+      for(int m = 0; m < N; ++m)
+        own[m] = own_p[m * N2];
+      // x-neighbours that are the previous / next cell of this workgroup are read from its
+      // LDS after the barrier (x-lines are the worst-coalesced global accesses)
+      in_block_l = slot > 0 && cc[0] > 0;
+      in_block_r = slot + 1 < CPB && cc[0] + 1 < a.topo.nc[0] && pos0 + slot + 1 < (int)a.n_cells;
+      Number w[3][2][N];
+#pragma unroll
+      for(int d = 0; d < 3; ++d)
+      {
+        int const gbase = d == 0 ? N * (p + N * q) : d == 1 ? p + N2 * q : p + N * q;
+        int const gstr  = d == 0 ? 1 : d == 1 ? N : N2;
+#pragma unroll
+        for(int s = 0; s < 2; ++s)
+        {
+          bool const skip = d == 0 && (s == 0 ? in_block_l : in_block_r);
+          if(!skip)
+          {
+            Number const * src = a.src + (size_t)(nb[d][s] >= 0 ? nb[d][s] : cell) * N3 + gbase;
+#pragma unroll
+            for(int i = 0; i < N; ++i)
+              w[d][s][i] = src[i * gstr];
+          }
+          else
+          {
+#pragma unroll
+            for(int i = 0; i < N; ++i)
+              w[d][s][i] = Number(0);
+          }
+        }
+      }
+#pragma unroll
+      for(int m = 0; m < N; ++m)
+        U[bz + L::PY * m] = own[m];
+#pragma unroll
+      for(int d = 0; d < 3; ++d)
+      {
+        Number const * tb = a.tables + d * TB;
+        wnb[d][0]         = w[d][0][N - 1];
+        dwnb[d][0]        = dot_line<N>(tb + 2 * N2 + N, w[d][0]);
+        wnb[d][1]         = w[d][1][0];
+        dwnb[d][1]        = dot_line<N>(tb + 2 * N2, w[d][1]);
+      }
+    }
   }
   __syncthreads();
   Number const * tx = a.tables;
   Number const * ty = a.tables + TB;
   Number const * tz = a.tables + 2 * TB;
 
+  if constexpr(mode != ApplyMode::diagonal)
+    if(active && (in_block_l || in_block_r))
+    {
+      Number w[N];
+      if(in_block_l)
+      {
+        load_line<N>(U - 3 * L::PC + bx, 1, w);
+        wnb[0][0]  = w[N - 1];
+        dwnb[0][0] = dot_line<N>(tx + 2 * N2 + N, w);
+      }
+      if(in_block_r)
+      {
+        load_line<N>(U + 3 * L::PC + bx, 1, w);
+        wnb[0][1]  = w[0];
+        dwnb[0][1] = dot_line<N>(tx + 2 * N2, w);
+      }
+    }
+
   // X: a_x = L_x u  -> A
   if(active)
   {
-    int const base = N * (p + N * q);
-    Number    u[N], o[N];
-    load_line<N>(U + base, 1, u);
-    int const    ty_[2] = {type[0][0], type[0][1]};
-    Number const ta_[2] = {tau[0][0], tau[0][1]};
-    line_laplacian<N, Number, mode>(tx, u, o, ty_, ta_, a.src + (nb[0][0] >= 0 ? nb[0][0] : 0) * N3 + base,
-                                    a.src + (nb[0][1] >= 0 ? nb[0][1] : 0) * N3 + base, 1,
-                                    nb[0][0] == cell);
-    store_line<N>(Abuf + base, 1, o);
+    Number u[N], o[N];
+    load_line<N>(U + bx, 1, u);
+    line_laplacian<N>(tx, u, o, type[0], tau[0], wnb[0], dwnb[0]);
+    store_line<N>(Abuf + bx, 1, o);
   }
   __syncthreads();
   // Y: A <- M_y a_x, B <- a_y = L_y u
   if(active)
   {
-    int const base = p + N2 * q;
-    Number    u[N], o[N], v[N], w[N];
-    load_line<N>(U + base, N, u);
-    int const    ty_[2] = {type[1][0], type[1][1]};
-    Number const ta_[2] = {tau[1][0], tau[1][1]};
-    line_laplacian<N, Number, mode>(ty, u, o, ty_, ta_, a.src + (nb[1][0] >= 0 ? nb[1][0] : 0) * N3 + base,
-                                    a.src + (nb[1][1] >= 0 ? nb[1][1] : 0) * N3 + base, N,
-                                    nb[1][0] == cell);
-    store_line<N>(Bbuf + base, N, o);
-    load_line<N>(Abuf + base, N, v);
+    Number u[N], o[N], v[N], w[N];
+    load_line<N>(U + by, L::PX, u);
+    line_laplacian<N>(ty, u, o, type[1], tau[1], wnb[1], dwnb[1]);
+    store_line<N>(Bbuf + by, L::PX, o);
+    load_line<N>(Abuf + by, L::PX, v);
     mat_apply<N>(ty, v, w);
-    store_line<N>(Abuf + base, N, w);
+    store_line<N>(Abuf + by, L::PX, w);
   }
   __syncthreads();
   // X2: A <- A + M_x a_y
   if(active)
   {
-    int const base = N * (p + N * q);
-    Number    v[N], w[N], r[N];
-    load_line<N>(Bbuf + base, 1, v);
+    Number v[N], w[N], r[N];
+    load_line<N>(Bbuf + bx, 1, v);
     mat_apply<N>(tx, v, w);
-    load_line<N>(Abuf + base, 1, r);
+    load_line<N>(Abuf + bx, 1, r);
 #pragma unroll
     for(int i = 0; i < N; ++i)
       r[i] += w[i];
-    store_line<N>(Abuf + base, 1, r);
+    store_line<N>(Abuf + bx, 1, r);
   }
   __syncthreads();
   // Z: out_partial = M_z A (registers), B <- a_z = L_z u
   Number outp[N];
   if(active)
   {
-    int const base = p + N * q;
-    Number    u[N], o[N], v[N];
-    load_line<N>(U + base, N2, u);
-    int const    ty_[2] = {type[2][0], type[2][1]};
-    Number const ta_[2] = {tau[2][0], tau[2][1]};
-    line_laplacian<N, Number, mode>(tz, u, o, ty_, ta_, a.src + (nb[2][0] >= 0 ? nb[2][0] : 0) * N3 + base,
-                                    a.src + (nb[2][1] >= 0 ? nb[2][1] : 0) * N3 + base, N2,
-                                    nb[2][0] == cell);
-    load_line<N>(Abuf + base, N2, v);
+    Number u[N], o[N], v[N];
+    load_line<N>(U + bz, L::PY, u);
+    line_laplacian<N>(tz, u, o, type[2], tau[2], wnb[2], dwnb[2]);
+    load_line<N>(Abuf + bz, L::PY, v);
     mat_apply<N>(tz, v, outp);
-    store_line<N>(Bbuf + base, N2, o);
+    store_line<N>(Bbuf + bz, L::PY, o);
   }
   __syncthreads();
   // X3: B <- M_x B
   if(active)
   {
-    int const base = N * (p + N * q);
-    Number    v[N], w[N];
-    load_line<N>(Bbuf + base, 1, v);
+    Number v[N], w[N];
+    load_line<N>(Bbuf + bx, 1, v);
     mat_apply<N>(tx, v, w);
-    store_line<N>(Bbuf + base, 1, w);
+    store_line<N>(Bbuf + bx, 1, w);
   }
   __syncthreads();
   // Y3: B <- M_y B
   if(active)
   {
-    int const base = p + N2 * q;
-    Number    v[N], w[N];
-    load_line<N>(Bbuf + base, N, v);
+    Number v[N], w[N];
+    load_line<N>(Bbuf + by, L::PX, v);
     mat_apply<N>(ty, v, w);
-    store_line<N>(Bbuf + base, N, w);
+    store_line<N>(Bbuf + by, L::PX, w);
   }
   __syncthreads();
   // out = out_partial + B along the z-lines
   if(active)
   {
-    int const base = p + N * q;
-    Number    v[N];
-    load_line<N>(Bbuf + base, N2, v);
+    Number v[N];
+    load_line<N>(Bbuf + bz, L::PY, v);
     if constexpr(mode == ApplyMode::diagonal)
     {
 #pragma unroll
       for(int k = 0; k < N; ++k)
-        if(base + k * N2 == a.probe)
-          a.dst[cell * N3 + a.probe] = outp[k] + v[k];
+        if(lt + k * N2 == a.probe)
+          a.dst[(size_t)cell * N3 + a.probe] = outp[k] + v[k];
     }
     else
     {
-      Number * out = a.dst + cell * N3 + base;
+      Number * out = a.dst + (size_t)cell * N3 + lt;
 #pragma unroll
       for(int k = 0; k < N; ++k)
       {
@@ -758,7 +845,7 @@ hipError_t launch_cartesian_n(SipgArgs<Number> const & a, hipStream_t stream)
 {
   constexpr int CPB    = cells_per_block(N);
   int const     blocks = (int)((a.n_cells + CPB - 1) / CPB);
-  size_t const  lds    = sizeof(Number) * CPB * 3 * N * N * N;
+  size_t const  lds    = sizeof(Number) * CPB * 3 * LdsLayout<N>::PC;
   hipLaunchKernelGGL((sipg_cartesian_kernel<N, Number, mode>), dim3(blocks), dim3(256), lds, stream,
                      a);
   return hipGetLastError();

@@ -1,5 +1,6 @@
 #include "laplace_operator.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace ExaDG
@@ -81,6 +82,8 @@ void LaplaceOperator<Number>::apply(ApplyMode mode, Number * dst, Number const *
   a.topo        = grid_->topology();
   a.n_cells     = grid_->n_cells();
   a.probe       = probe;
+  // one tile of y-rows per XCD
+  a.tile_rows = std::max(1, (grid_->topology().nc[1] + 7) / 8);
   hipError_t const e = grid_->affine() ? launch_sipg_cartesian<Number>(data_.degree, mode, a, s)
                                        : launch_sipg_general<Number>(data_.degree, mode, a, s);
   if(e != hipSuccess)

@@ -59,4 +59,31 @@ __host__ __device__ inline int64_t face_id(BoxTopology const & t, int const * cc
   return t.face_base[d] + fl + (int64_t)t.layers[d] * (cc[a] + (int64_t)t.nc[a] * cc[b]);
 }
 
+// Cell visited at traversal position idx: tiles of `tile_rows` y-rows, each walked with x
+// fastest, then y within the tile, then z, so that all six face neighbours of a cell are
+// visited within about nx*tile_rows positions of it.
+__host__ __device__ inline int64_t traversal_cell(BoxTopology const & t, int64_t idx, int tile_rows)
+{
+  int const     nx   = t.nc[0], ny = t.nc[1], nz = t.nc[2];
+  int64_t const full = (int64_t)nx * nz * tile_rows;
+  int64_t const tile = idx / full;
+  int64_t const r    = idx - tile * full;
+  int const     rows = ny - (int)tile * tile_rows < tile_rows ? ny - (int)tile * tile_rows : tile_rows;
+  int const     x    = (int)(r % nx);
+  int64_t const r2   = r / nx;
+  int const     yi   = (int)(r2 % rows);
+  int const     z    = (int)(r2 / rows);
+  return x + (int64_t)nx * ((int64_t)tile * tile_rows + yi + (int64_t)ny * z);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (block b runs on XCD
+// b % 8). Renumber them so that each XCD receives one contiguous range of logical blocks and
+// therefore one contiguous part of the traversal (its own L2 serves the neighbour reads).
+__host__ __device__ inline int64_t xcd_logical_block(int64_t b, int64_t n_blocks)
+{
+  int64_t const q = n_blocks / 8, r = n_blocks % 8;
+  int64_t const x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
+}
+
 }  // namespace ExaDG
