@@ -41,6 +41,58 @@ class BCDesc(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32 * 6)]
 
 
+class MGDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_dg_levels", ctypes.c_int32),
+        ("dg_degrees", ctypes.c_int32 * 16),
+        ("coarse_q1", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("smoother_degree", ctypes.c_int32),
+        ("smoothing_range", ctypes.c_double),
+        ("eig_cg_iterations", ctypes.c_int32),
+        ("coarse_solver", ctypes.c_int32),
+        ("coarse_rel_tol", ctypes.c_double),
+        ("coarse_max_iter", ctypes.c_int32),
+        ("operator_is_singular", ctypes.c_int32),
+    ]
+
+
+class SolverData(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int32), ("abs_tol", ctypes.c_double),
+                ("rel_tol", ctypes.c_double), ("rel_mode", ctypes.c_int32)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("not_finite", ctypes.c_int32), ("r0", ctypes.c_double), ("r_final", ctypes.c_double),
+                ("b_norm", ctypes.c_double), ("rho", ctypes.c_double), ("n10", ctypes.c_double)]
+
+
+PRECOND_NONE, PRECOND_POINT_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
+COARSE_CG, COARSE_CHEBYSHEV = 0, 1
+
+
+def mg_desc(degree, degrees=None, precision=FP32, coarse_q1=True, coarse_solver=COARSE_CG,
+            coarse_rel_tol=1e-3, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=20,
+            singular=False):
+    """MultigridData; default p-sequence Bisect (k, k/2, ..., 1) + continuous Q1 level."""
+    d = MGDesc()
+    check(lib().exadg_mg_desc_default(degree, ctypes.byref(d)))
+    if degrees is not None:
+        d.n_dg_levels = len(degrees)
+        for i, p in enumerate(degrees):
+            d.dg_degrees[i] = p
+    d.precision = precision
+    d.coarse_q1 = int(coarse_q1)
+    d.coarse_solver = coarse_solver
+    d.coarse_rel_tol = coarse_rel_tol
+    d.smoother_degree = smoother_degree
+    d.smoothing_range = smoothing_range
+    d.eig_cg_iterations = eig_cg_iterations
+    d.operator_is_singular = int(singular)
+    return d
+
+
 def mesh_desc(n_cells, lower=0.0, upper=1.0, deformation=0.0, frequency=1, mapping_degree=1,
               periodic=(0, 0, 0)):
     if isinstance(n_cells, int):
@@ -84,6 +136,15 @@ def lib():
             "exadg_laplace_calculate_inverse_diagonal": [vp, vp, vp],
             "exadg_laplace_calculate_diagonal": [vp, vp, vp],
             "exadg_laplace_rhs_sine": [vp, ctypes.c_double, vp, vp],
+            "exadg_mg_desc_default": [ctypes.c_int, ctypes.POINTER(MGDesc)],
+            "exadg_preconditioner_create": [vp, ctypes.c_int, ctypes.POINTER(MGDesc),
+                                            ctypes.POINTER(vp), vp],
+            "exadg_preconditioner_vmult": [vp, vp, vp, vp],
+            "exadg_preconditioner_update": [vp, vp],
+            "exadg_preconditioner_destroy": [vp],
+            "exadg_preconditioner_mg_info": [vp, ip, ctypes.POINTER(ctypes.c_double), ip, ip],
+            "exadg_cg_solve": [vp, vp, vp, vp, ctypes.POINTER(SolverData), ctypes.POINTER(SolveInfo),
+                               vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -170,3 +231,62 @@ class LaplaceOperator:
         import torch
         assert b.is_cuda and b.dtype == torch.float64 and b.numel() == self.n
         check(lib().exadg_laplace_rhs_sine(self._h, omega, _ptr(b), _stream(stream)))
+
+
+class Preconditioner:
+    """PreconditionerBase: None / PointJacobi / Multigrid (FP32 or FP64 V-cycle)."""
+
+    def __init__(self, op, kind, mg=None, stream=None):
+        self.kind = kind
+        self._h = ctypes.c_void_p()
+        check(lib().exadg_preconditioner_create(op.handle, kind,
+                                                ctypes.byref(mg) if mg is not None else None,
+                                                ctypes.byref(self._h), _stream(stream)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().exadg_preconditioner_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def vmult(self, dst, src, stream=None):
+        check(lib().exadg_preconditioner_vmult(self._h, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def update(self, stream=None):
+        check(lib().exadg_preconditioner_update(self._h, _stream(stream)))
+
+    def mg_info(self):
+        nl = ctypes.c_int()
+        lmax = (ctypes.c_double * 32)()
+        deg = (ctypes.c_int * 32)()
+        cits = ctypes.c_int()
+        check(lib().exadg_preconditioner_mg_info(self._h, ctypes.byref(nl), lmax, deg,
+                                                 ctypes.byref(cits)))
+        L = nl.value
+        return dict(n_levels=L, lambda_max=[lmax[i] for i in range(L)],
+                    chebyshev_degree=[deg[i] for i in range(L)],
+                    last_coarse_iterations=cits.value)
+
+
+def cg_solve(op, b, x, prec=None, rel_tol=1e-10, abs_tol=1e-20, max_iter=10000, rel_mode=0,
+             stream=None, raise_on_error=True):
+    """Krylov::KrylovSolver::solve (CG) on the FP64 operator; x holds the initial guess."""
+    d = SolverData(max_iter, abs_tol, rel_tol, rel_mode)
+    info = SolveInfo()
+    rc = lib().exadg_cg_solve(op.handle, prec.handle if prec is not None else None, _ptr(b), _ptr(x),
+                              ctypes.byref(d), ctypes.byref(info), _stream(stream))
+    out = dict(iterations=info.iterations, converged=bool(info.converged),
+               not_finite=bool(info.not_finite), r0=info.r0, r_final=info.r_final,
+               b_norm=info.b_norm, rho=info.rho, n10=info.n10, status=rc)
+    if rc != EXADG_OK and raise_on_error:
+        raise ExaDGError(rc, lib().exadg_last_error().decode())
+    return out

@@ -1,11 +1,14 @@
 // extern "C" boundary: every entry point catches, records the message and returns a status.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
 
 #include "../../include/exadg_amd.h"
 #include "common.h"
+#include "krylov.h"
 #include "laplace_operator.h"
+#include "multigrid.h"
 
 namespace
 {
@@ -53,7 +56,44 @@ void require(bool cond, char const * msg)
 struct exadg_laplace
 {
   std::unique_ptr<ExaDG::Poisson::LaplaceOperatorBase> op;
+  std::unique_ptr<ExaDG::Krylov::SolverCG>             cg;
 };
+
+struct exadg_preconditioner
+{
+  int                                        kind = 0;
+  std::unique_ptr<ExaDG::PreconditionerBase> p;
+};
+
+namespace
+{
+ExaDG::Poisson::LaplaceOperator<double> & fp64_operator(exadg_laplace * op)
+{
+  auto * d = dynamic_cast<ExaDG::Poisson::LaplaceOperator<double> *>(op->op.get());
+  if(!d)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "the outer solver needs an FP64 operator");
+  return *d;
+}
+
+template<typename Number>
+void mg_info(ExaDG::PreconditionerBase const * p, int * n_levels, double * lmax, int * degs,
+             int * coarse_its)
+{
+  auto const & mg = static_cast<ExaDG::MultigridPreconditioner<Number> const &>(*p);
+  if(n_levels)
+    *n_levels = mg.n_levels();
+  for(int l = 0; l < mg.n_levels(); ++l)
+  {
+    auto const & sm = l == 0 ? mg.coarse_chebyshev() : mg.smoother(l);
+    if(lmax)
+      lmax[l] = sm.lambda_max_estimate;
+    if(degs)
+      degs[l] = sm.degree;
+  }
+  if(coarse_its)
+    *coarse_its = mg.last_coarse_iterations();
+}
+}  // namespace
 
 extern "C" {
 
@@ -138,6 +178,160 @@ int exadg_laplace_rhs_sine(exadg_laplace * op, double omega, double * b, void * 
     require(op, "null operator");
     op->op->rhs_sine(omega, b, stream_of(stream));
   });
+}
+
+int exadg_mg_desc_default(int degree, exadg_mg_desc * out)
+{
+  return guarded([&] {
+    require(out != nullptr, "null argument");
+    require(degree >= 1 && degree <= 8, "degree must be in [1,8]");
+    exadg_mg_desc d{};
+    int           p = degree;
+    while(true)
+    {
+      d.dg_degrees[d.n_dg_levels++] = p;
+      if(p == 1)
+        break;
+      p = std::max(p / 2, 1);
+    }
+    d.coarse_q1            = 1;
+    d.precision            = EXADG_FP32;
+    d.smoother_degree      = 5;
+    d.smoothing_range      = 20.0;
+    d.eig_cg_iterations    = 20;
+    d.coarse_solver        = EXADG_COARSE_CG;
+    d.coarse_rel_tol       = 1e-3;
+    d.coarse_max_iter      = 10000;
+    d.operator_is_singular = 0;
+    *out                   = d;
+  });
+}
+
+int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_desc * mg,
+                                exadg_preconditioner ** out, void * stream)
+{
+  return guarded([&] {
+    require(op && out, "null argument");
+    auto &     A = fp64_operator(op);
+    auto       h = std::make_unique<exadg_preconditioner>();
+    hipStream_t const s = stream_of(stream);
+    h->kind             = kind;
+    if(kind == EXADG_PRECOND_NONE)
+    {
+    }
+    else if(kind == EXADG_PRECOND_POINT_JACOBI)
+    {
+      h->p = std::make_unique<ExaDG::JacobiPreconditioner>(A);
+      h->p->update(s);
+    }
+    else if(kind == EXADG_PRECOND_MULTIGRID)
+    {
+      require(mg != nullptr, "multigrid preconditioner needs an exadg_mg_desc");
+      require(mg->n_dg_levels >= 1 && mg->n_dg_levels <= 16, "n_dg_levels out of range");
+      require(mg->dg_degrees[0] == A.degree(), "dg_degrees[0] must equal the operator degree");
+      ExaDG::MultigridData d;
+      d.degrees.assign(mg->dg_degrees, mg->dg_degrees + mg->n_dg_levels);
+      d.coarse_q1            = mg->coarse_q1 != 0;
+      d.smoother_degree      = mg->smoother_degree;
+      d.smoothing_range      = mg->smoothing_range;
+      d.eig_cg_iterations    = mg->eig_cg_iterations;
+      d.coarse_solver        = mg->coarse_solver == EXADG_COARSE_CHEBYSHEV ? ExaDG::CoarseSolver::chebyshev
+                                                                           : ExaDG::CoarseSolver::cg;
+      d.coarse_rel_tol       = mg->coarse_rel_tol;
+      d.coarse_max_iter      = mg->coarse_max_iter;
+      d.operator_is_singular = mg->operator_is_singular != 0;
+      require(d.smoother_degree >= 1 && d.smoothing_range > 1.0 && d.eig_cg_iterations >= 1,
+              "invalid smoother data");
+      require(d.coarse_rel_tol > 0.0 && d.coarse_rel_tol < 1.0 && d.coarse_max_iter >= 1,
+              "invalid coarse solver data");
+      h->p = ExaDG::create_multigrid(A.grid_ptr(), d, A.ip_factor(), mg->precision, s);
+      h->p->update(s);
+    }
+    else
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "unknown preconditioner kind");
+    h->kind = kind | (kind == EXADG_PRECOND_MULTIGRID && mg->precision == EXADG_FP32 ? 0x100 : 0);
+    *out    = h.release();
+  });
+}
+
+int exadg_preconditioner_vmult(exadg_preconditioner * p, double * dst, const double * src,
+                               void * stream)
+{
+  return guarded([&] {
+    require(p && dst && src, "null argument");
+    require(dst != src, "dst and src must not alias");
+    require(p->p != nullptr, "EXADG_PRECOND_NONE has no vmult");
+    p->p->vmult(dst, src, stream_of(stream));
+  });
+}
+
+int exadg_preconditioner_update(exadg_preconditioner * p, void * stream)
+{
+  return guarded([&] {
+    require(p, "null argument");
+    if(p->p)
+      p->p->update(stream_of(stream));
+  });
+}
+
+int exadg_preconditioner_destroy(exadg_preconditioner * p)
+{
+  return guarded([&] { delete p; });
+}
+
+int exadg_preconditioner_mg_info(const exadg_preconditioner * p, int * n_levels, double * lmax,
+                                 int * cheb_degree, int * coarse_its)
+{
+  return guarded([&] {
+    require(p, "null argument");
+    require((p->kind & 0xff) == EXADG_PRECOND_MULTIGRID, "not a multigrid preconditioner");
+    if(p->kind & 0x100)
+      mg_info<float>(p->p.get(), n_levels, lmax, cheb_degree, coarse_its);
+    else
+      mg_info<double>(p->p.get(), n_levels, lmax, cheb_degree, coarse_its);
+  });
+}
+
+int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
+                   const exadg_solver_data * data, exadg_solve_info * info, void * stream)
+{
+  int                           status = EXADG_OK;
+  ExaDG::Krylov::SolveInfo      si;
+  int const rc = guarded([&] {
+    require(op && b && x && data, "null argument");
+    require(data->max_iter >= 0 && data->rel_tol >= 0 && data->abs_tol >= 0, "invalid solver data");
+    auto & A = fp64_operator(op);
+    if(!op->cg)
+      op->cg = std::make_unique<ExaDG::Krylov::SolverCG>();
+    ExaDG::Krylov::SolverData sd;
+    sd.max_iter = data->max_iter;
+    sd.abs_tol  = data->abs_tol;
+    sd.rel_tol  = data->rel_tol;
+    sd.rel_mode = data->rel_mode;
+    si          = op->cg->solve(A, prec ? prec->p.get() : nullptr, b, x, sd, stream_of(stream));
+    if(si.not_finite)
+    {
+      g_last_error = "Last iteration step contained NaN or Inf values.";
+      status       = EXADG_ERR_NOT_FINITE;
+    }
+    else if(!si.converged)
+    {
+      g_last_error = "CG reached the iteration limit";
+      status       = EXADG_ERR_NO_CONVERGE;
+    }
+  });
+  if(info)
+  {
+    info->iterations = si.iterations;
+    info->converged  = si.converged ? 1 : 0;
+    info->not_finite = si.not_finite ? 1 : 0;
+    info->r0         = si.r0;
+    info->r_final    = si.r_final;
+    info->b_norm     = si.b_norm;
+    info->rho        = si.rho;
+    info->n10        = si.n10;
+  }
+  return rc != EXADG_OK ? rc : status;
 }
 
 }  // extern "C"

@@ -35,6 +35,8 @@ public:
   virtual void    calculate_diagonal(void * diag, hipStream_t s) const         = 0;
   virtual void    calculate_inverse_diagonal(void * diag, hipStream_t s) const = 0;
   virtual void    rhs_sine(double omega, double * b, hipStream_t s) const      = 0;
+  virtual std::shared_ptr<StructuredGrid const> grid_ptr() const               = 0;
+  virtual double                                ip_factor() const              = 0;
 };
 
 template<typename Number>
@@ -78,6 +80,8 @@ public:
   void rhs_sine(double omega, double * b, hipStream_t s) const override;
 
   StructuredGrid const & grid() const { return *grid_; }
+  std::shared_ptr<StructuredGrid const> grid_ptr() const override { return grid_; }
+  double                                ip_factor() const override { return data_.IP_factor; }
   ShapeInfo1D const &    shape() const { return shape_; }
   GeometryData const &   geometry_host() const { return geo_; }
 

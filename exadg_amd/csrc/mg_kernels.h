@@ -1,0 +1,51 @@
+// Multigrid kernels: polynomial (p) transfer between DG levels on the same mesh, the DG-Q1 <->
+// CG-Q1 (c) transfer, and the continuous Q1 coarse operator.
+//
+// Reference: MultigridTransfer (solvers_and_preconditioners/multigrid/transfer.cpp:28-96) ->
+// MGTwoLevelTransfer::reinit_polynomial_transfer; the continuous coarse level of the hybrid
+// multigrid (multigrid_preconditioner_base.cpp:191-226) with Dirichlet constraints
+// (multigrid/constraints.h:109-194) acting as identity rows (operator_base.cpp:299-309).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "topology.h"
+
+namespace ExaDG
+{
+// Q1 node numbering of the structured box: (n_d + 1) nodes per non-periodic direction,
+// n_d nodes per periodic direction (node n_d is node 0).
+struct Q1Topology
+{
+  int nc[3];
+  int nn[3];        // nodes per direction
+  int periodic[3];
+  int dirichlet[6]; // side constrained (Dirichlet, non-periodic)
+};
+
+// dst_f += (P x P x P) src_c per cell, P = nf x nc interpolation matrix (row-major)
+template<typename Number>
+hipError_t launch_p_prolongate_add(Number * dst_f, Number const * src_c, Number const * P, int nf,
+                                   int nc, int64_t n_cells, hipStream_t s);
+// dst_c += (P^T x P^T x P^T) src_f per cell
+template<typename Number>
+hipError_t launch_p_restrict_add(Number * dst_c, Number const * src_f, Number const * P, int nf,
+                                 int nc, int64_t n_cells, hipStream_t s);
+// DG1 (8 vertex values per cell, lexicographic) += CG1 node values (constrained nodes read as 0)
+template<typename Number>
+hipError_t launch_c_prolongate_add(Number * dst_dg, Number const * src_cg, BoxTopology const & t,
+                                   Q1Topology const & q, hipStream_t s);
+// CG1 node += sum over the cells sharing it of the DG1 vertex values (constrained nodes skipped)
+template<typename Number>
+hipError_t launch_c_restrict_add(Number * dst_cg, Number const * src_dg, BoxTopology const & t,
+                                 Q1Topology const & q, hipStream_t s);
+// Q1 operator: dst = A src with per-cell 8x8 element matrices (one shared matrix when
+// `shared_matrix`); constrained rows are identity, constrained columns read as zero.
+// mode 0: vmult, 1: diagonal (src ignored)
+template<typename Number>
+hipError_t launch_q1_apply(Number * dst, Number const * src, Number const * cell_matrices,
+                           int shared_matrix, BoxTopology const & t, Q1Topology const & q,
+                           int mode, hipStream_t s);
+
+}  // namespace ExaDG

@@ -1,0 +1,279 @@
+// Multigrid transfer and coarse-level kernels (see mg_kernels.h).
+#include <hip/hip_runtime.h>
+
+#include "../../include/exadg_amd.h"
+#include "mg_kernels.h"
+
+namespace ExaDG
+{
+namespace
+{
+// ---------------------------------------------------------------- p-transfer
+//
+// One cell per group of nf^2 lanes, three sum-factorized 1D interpolation sweeps through LDS.
+// PROLONG: input nc^3 -> output nf^3 with M = P; else input nf^3 -> output nc^3 with M = P^T.
+template<typename Number, bool PROLONG>
+__global__ void __launch_bounds__(256)
+  p_transfer_kernel(Number * dst, Number const * src, Number const * P, int nf, int nc,
+                    int64_t n_cells, int cpb)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Number *      sm     = reinterpret_cast<Number *>(smem_raw);
+  int const     nf3    = nf * nf * nf;
+  int const     tpc    = nf * nf;
+  int const     slot   = threadIdx.x / tpc;
+  int const     lt     = threadIdx.x - slot * tpc;
+  int64_t const cell   = (int64_t)blockIdx.x * cpb + slot;
+  bool const    active = slot < cpb && cell < n_cells;
+  Number *      b0     = sm + (active ? slot : 0) * 2 * nf3;
+  Number *      b1     = b0 + nf3;
+  int const     nin = PROLONG ? nc : nf, nout = PROLONG ? nf : nc;
+  int const     nin3 = nin * nin * nin, nout3 = nout * nout * nout;
+  auto          M = [&](int a, int i) { return PROLONG ? P[a * nc + i] : P[i * nc + a]; };
+
+  if(active)
+    for(int e = lt; e < nin3; e += tpc)
+      b0[e] = src[cell * nin3 + e];
+  __syncthreads();
+  if(active)
+    for(int task = lt; task < nin * nin; task += tpc)
+    {
+      int const j = task % nin, k = task / nin;
+      for(int a = 0; a < nout; ++a)
+      {
+        Number s = 0;
+        for(int i = 0; i < nin; ++i)
+          s += M(a, i) * b0[i + nin * (j + nin * k)];
+        b1[a + nout * (j + nin * k)] = s;
+      }
+    }
+  __syncthreads();
+  if(active)
+    for(int task = lt; task < nout * nin; task += tpc)
+    {
+      int const a = task % nout, k = task / nout;
+      for(int b = 0; b < nout; ++b)
+      {
+        Number s = 0;
+        for(int j = 0; j < nin; ++j)
+          s += M(b, j) * b1[a + nout * (j + nin * k)];
+        b0[a + nout * (b + nout * k)] = s;
+      }
+    }
+  __syncthreads();
+  if(active)
+    for(int task = lt; task < nout * nout; task += tpc)
+    {
+      int const a = task % nout, b = task / nout;
+      for(int c = 0; c < nout; ++c)
+      {
+        Number s = 0;
+        for(int k = 0; k < nin; ++k)
+          s += M(c, k) * b0[a + nout * (b + nout * k)];
+        dst[cell * nout3 + a + nout * (b + nout * c)] += s;
+      }
+    }
+}
+
+// ---------------------------------------------------------------- Q1 topology helpers
+
+__device__ __forceinline__ bool q1_constrained(Q1Topology const & q, int const * nd)
+{
+#pragma unroll
+  for(int d = 0; d < 3; ++d)
+    if(!q.periodic[d] && ((nd[d] == 0 && q.dirichlet[2 * d]) || (nd[d] == q.nc[d] && q.dirichlet[2 * d + 1])))
+      return true;
+  return false;
+}
+
+__device__ __forceinline__ int64_t q1_node_id(Q1Topology const & q, int const * nd)
+{
+  return nd[0] + (int64_t)q.nn[0] * (nd[1] + (int64_t)q.nn[1] * nd[2]);
+}
+
+// node coordinates of vertex v (bits x,y,z) of the cell at cc, periodic wrap
+__device__ __forceinline__ void q1_vertex_node(Q1Topology const & q, int const * cc, int v, int * nd)
+{
+#pragma unroll
+  for(int d = 0; d < 3; ++d)
+  {
+    int const c = cc[d] + ((v >> d) & 1);
+    nd[d]       = (q.periodic[d] && c == q.nc[d]) ? 0 : c;
+  }
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(256)
+  c_prolongate_kernel(Number * dst_dg, Number const * src_cg, BoxTopology t, Q1Topology q,
+                      int64_t n)
+{
+  int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n)
+    return;
+  int64_t const cell = i / 8;
+  int const     v    = (int)(i % 8);
+  int           cc[3], nd[3];
+  cell_coords(t, cell, cc);
+  q1_vertex_node(q, cc, v, nd);
+  if(!q1_constrained(q, nd))
+    dst_dg[i] += src_cg[q1_node_id(q, nd)];
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(256)
+  c_restrict_kernel(Number * dst_cg, Number const * src_dg, BoxTopology t, Q1Topology q, int64_t n)
+{
+  int64_t const node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(node >= n)
+    return;
+  int nd[3];
+  nd[0]             = (int)(node % q.nn[0]);
+  int64_t const r   = node / q.nn[0];
+  nd[1]             = (int)(r % q.nn[1]);
+  nd[2]             = (int)(r / q.nn[1]);
+  if(q1_constrained(q, nd))
+    return;
+  Number acc = 0;
+  // the (up to) 8 cells sharing the node: cell coordinate nd-1 (vertex bit 1) or nd (bit 0)
+  for(int v = 0; v < 8; ++v)
+  {
+    int  cc[3];
+    bool ok = true;
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+      int c = nd[d] - ((v >> d) & 1);
+      if(c < 0)
+        c = q.periodic[d] ? c + q.nc[d] : -1;
+      if(c < 0 || c >= q.nc[d])
+        ok = false;
+      cc[d] = c;
+    }
+    if(ok)
+      acc += src_dg[cell_id(t, cc) * 8 + v];
+  }
+  dst_cg[node] += acc;
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(256)
+  q1_apply_kernel(Number * dst, Number const * src, Number const * mats, int shared, BoxTopology t,
+                  Q1Topology q, int mode, int64_t n)
+{
+  int64_t const node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(node >= n)
+    return;
+  int nd[3];
+  nd[0]           = (int)(node % q.nn[0]);
+  int64_t const r = node / q.nn[0];
+  nd[1]           = (int)(r % q.nn[1]);
+  nd[2]           = (int)(r / q.nn[1]);
+  if(q1_constrained(q, nd))
+  {  // identity row
+    dst[node] = mode == 0 ? src[node] : Number(1);
+    return;
+  }
+  Number acc = 0;
+  for(int v = 0; v < 8; ++v)
+  {
+    int  cc[3];
+    bool ok = true;
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+      int c = nd[d] - ((v >> d) & 1);
+      if(c < 0)
+        c = q.periodic[d] ? c + q.nc[d] : -1;
+      if(c < 0 || c >= q.nc[d])
+        ok = false;
+      cc[d] = c;
+    }
+    if(!ok)
+      continue;
+    int64_t const  cell = cell_id(t, cc);
+    Number const * K    = mats + (shared ? 0 : cell * 64) + v * 8;
+    if(mode == 1)
+      acc += K[v];
+    else
+      for(int m = 0; m < 8; ++m)
+      {
+        int nm[3];
+        q1_vertex_node(q, cc, m, nm);
+        if(!q1_constrained(q, nm))
+          acc += K[m] * src[q1_node_id(q, nm)];
+      }
+  }
+  dst[node] = acc;
+}
+
+inline int blocks_for(int64_t n) { return (int)((n + 255) / 256); }
+}  // namespace
+
+template<typename Number>
+hipError_t launch_p_prolongate_add(Number * dst_f, Number const * src_c, Number const * P, int nf,
+                                   int nc, int64_t n_cells, hipStream_t s)
+{
+  int const    cpb = 256 / (nf * nf);
+  size_t const lds = sizeof(Number) * cpb * 2 * nf * nf * nf;
+  hipLaunchKernelGGL((p_transfer_kernel<Number, true>), dim3((n_cells + cpb - 1) / cpb), dim3(256),
+                     lds, s, dst_f, src_c, P, nf, nc, n_cells, cpb);
+  return hipGetLastError();
+}
+
+template<typename Number>
+hipError_t launch_p_restrict_add(Number * dst_c, Number const * src_f, Number const * P, int nf,
+                                 int nc, int64_t n_cells, hipStream_t s)
+{
+  int const    cpb = 256 / (nf * nf);
+  size_t const lds = sizeof(Number) * cpb * 2 * nf * nf * nf;
+  hipLaunchKernelGGL((p_transfer_kernel<Number, false>), dim3((n_cells + cpb - 1) / cpb), dim3(256),
+                     lds, s, dst_c, src_f, P, nf, nc, n_cells, cpb);
+  return hipGetLastError();
+}
+
+template<typename Number>
+hipError_t launch_c_prolongate_add(Number * dst_dg, Number const * src_cg, BoxTopology const & t,
+                                   Q1Topology const & q, hipStream_t s)
+{
+  int64_t const n = (int64_t)t.nc[0] * t.nc[1] * t.nc[2] * 8;
+  hipLaunchKernelGGL((c_prolongate_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, dst_dg,
+                     src_cg, t, q, n);
+  return hipGetLastError();
+}
+
+template<typename Number>
+hipError_t launch_c_restrict_add(Number * dst_cg, Number const * src_dg, BoxTopology const & t,
+                                 Q1Topology const & q, hipStream_t s)
+{
+  int64_t const n = (int64_t)q.nn[0] * q.nn[1] * q.nn[2];
+  hipLaunchKernelGGL((c_restrict_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, dst_cg,
+                     src_dg, t, q, n);
+  return hipGetLastError();
+}
+
+template<typename Number>
+hipError_t launch_q1_apply(Number * dst, Number const * src, Number const * mats, int shared,
+                           BoxTopology const & t, Q1Topology const & q, int mode, hipStream_t s)
+{
+  int64_t const n = (int64_t)q.nn[0] * q.nn[1] * q.nn[2];
+  hipLaunchKernelGGL((q1_apply_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, dst, src, mats,
+                     shared, t, q, mode, n);
+  return hipGetLastError();
+}
+
+#define EXADG_INST(T)                                                                              \
+  template hipError_t launch_p_prolongate_add<T>(T *, T const *, T const *, int, int, int64_t,     \
+                                                 hipStream_t);                                     \
+  template hipError_t launch_p_restrict_add<T>(T *, T const *, T const *, int, int, int64_t,       \
+                                               hipStream_t);                                       \
+  template hipError_t launch_c_prolongate_add<T>(T *, T const *, BoxTopology const &,              \
+                                                 Q1Topology const &, hipStream_t);                 \
+  template hipError_t launch_c_restrict_add<T>(T *, T const *, BoxTopology const &,                \
+                                               Q1Topology const &, hipStream_t);                   \
+  template hipError_t launch_q1_apply<T>(T *, T const *, T const *, int, BoxTopology const &,      \
+                                         Q1Topology const &, int, hipStream_t);
+EXADG_INST(double)
+EXADG_INST(float)
+#undef EXADG_INST
+
+}  // namespace ExaDG

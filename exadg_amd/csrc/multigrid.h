@@ -1,0 +1,190 @@
+// Hybrid p-multigrid preconditioner (DG_k -> ... -> DG_1 -> continuous Q1) with
+// Chebyshev-Jacobi smoothing, run in FP32 or FP64 inside an FP64 outer Krylov solver.
+//
+// Reference interfaces: MultigridPreconditionerBase::vmult/update
+// (solvers_and_preconditioners/multigrid/multigrid_preconditioner_base.cpp:55-95,673-683),
+// MultigridAlgorithm::vmult/v_cycle (multigrid/multigrid_algorithm.h:87-108,172-240),
+// ChebyshevSmoother (multigrid/smoothers/chebyshev_smoother.h:37-227) over deal.II's
+// PreconditionChebyshev, MGCoarseKrylov / MGCoarseChebyshev (multigrid/coarse_grid_solvers.h),
+// MultigridData (multigrid/multigrid_parameters.h:39-313), PreconditionerBase
+// (preconditioners/preconditioner_base.h:33-67).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "grid.h"
+#include "laplace_operator.h"
+#include "mg_kernels.h"
+#include "vector_ops.h"
+
+namespace ExaDG
+{
+// PreconditionerBase<double>: the FP64 "apply_inverse" seen by the outer solver
+class PreconditionerBase
+{
+public:
+  virtual ~PreconditionerBase() = default;
+  virtual void vmult(double * dst, double const * src, hipStream_t s) = 0;
+  virtual void update(hipStream_t s)                                  = 0;
+};
+
+// point Jacobi (jacobi_preconditioner.h:33-82)
+class JacobiPreconditioner : public PreconditionerBase
+{
+public:
+  explicit JacobiPreconditioner(Poisson::LaplaceOperator<double> const & op);
+  void vmult(double * dst, double const * src, hipStream_t s) override;
+  void update(hipStream_t s) override;
+
+private:
+  Poisson::LaplaceOperator<double> const & op_;
+  DeviceVector<double>                     inv_diag_;
+};
+
+enum class CoarseSolver : int
+{
+  cg        = 0,  // MGCoarseKrylov: CG + point Jacobi to a relative tolerance
+  chebyshev = 1   // MGCoarseChebyshev: Chebyshev-Jacobi with degree from the tolerance
+};
+
+struct MultigridData
+{
+  std::vector<int> degrees = {3, 1};     // DG degrees, fine to coarse (degrees[0] = k)
+  bool             coarse_q1 = true;     // append a continuous Q1 level via a c-transfer
+  int              smoother_degree = 5;  // SmootherData::iterations
+  double           smoothing_range = 20; // SmootherData::smoothing_range
+  int              eig_cg_iterations = 20;
+  CoarseSolver     coarse_solver   = CoarseSolver::cg;
+  double           coarse_rel_tol  = 1e-3;
+  int              coarse_max_iter = 10000;
+  bool             operator_is_singular = false;
+};
+
+// work vectors of a PCG on one level (reused across solves)
+template<typename Number>
+struct PcgWorkspace
+{
+  DeviceVector<Number> r, z, p, v;
+  void reinit(int64_t n)
+  {
+    r.reinit(n);
+    z.reinit(n);
+    p.reinit(n);
+    v.reinit(n);
+  }
+};
+
+template<typename Number>
+class LevelOperator
+{
+public:
+  virtual ~LevelOperator() = default;
+  virtual int64_t n() const                                                 = 0;
+  virtual void    vmult(Number * dst, Number const * src, hipStream_t s) const = 0;
+  virtual void    inverse_diagonal(Number * d, hipStream_t s) const           = 0;
+  virtual bool    continuous() const { return false; }
+  // mask of constrained entries (1 constrained, 0 free) on the host; empty if none
+  virtual std::vector<char> constrained_mask() const { return {}; }
+};
+
+// continuous Q1 operator with strong Dirichlet constraints on the structured box
+template<typename Number>
+class Q1Operator : public LevelOperator<Number>
+{
+public:
+  explicit Q1Operator(StructuredGrid const & grid);
+  int64_t n() const override { return n_nodes_; }
+  void    vmult(Number * dst, Number const * src, hipStream_t s) const override;
+  void    inverse_diagonal(Number * d, hipStream_t s) const override;
+  bool    continuous() const override { return true; }
+  std::vector<char>  constrained_mask() const override;
+  Q1Topology const & q1_topology() const { return q_; }
+  std::vector<double> const & element_matrices() const { return mats_host_; }
+
+private:
+  BoxTopology          topo_;
+  Q1Topology           q_;
+  int64_t              n_nodes_;
+  int                  shared_;
+  std::vector<double>  mats_host_;
+  DeviceBuffer<Number> mats_;
+};
+
+// Chebyshev polynomial smoother with point-Jacobi inner preconditioner (deal.II
+// PreconditionChebyshev semantics: vmult assumes a zero initial guess, step does not)
+template<typename Number>
+class ChebyshevSmoother
+{
+public:
+  void setup(LevelOperator<Number> const & op, int degree, double smoothing_range,
+             int eig_cg_iterations, hipStream_t s);
+  // coarse-grid variant: interval [lambda_min, 1.1 lambda_max] from a CG estimate to 1e-5
+  void setup_coarse(LevelOperator<Number> const & op, double rel_tol, bool singular, hipStream_t s);
+  // x: current solution buffer, x_alt: second buffer; both may be swapped on return
+  void vmult(Number *& x, Number *& x_alt, Number const * b, hipStream_t s) const;
+  void step(Number *& x, Number *& x_alt, Number const * b, hipStream_t s) const;
+
+  double lambda_max_estimate = 0, lambda_min_estimate = 0, theta = 0, delta = 0;
+  int    degree = 0, cg_iterations = 0;
+
+private:
+  LevelOperator<Number> const * op_ = nullptr;
+  DeviceVector<Number>          inv_diag_;
+  mutable DeviceVector<Number>  t_;
+};
+
+template<typename Number>
+class MultigridPreconditioner : public PreconditionerBase
+{
+public:
+  MultigridPreconditioner(std::shared_ptr<StructuredGrid const> grid, MultigridData const & data,
+                          double ip_factor, hipStream_t s);
+  void vmult(double * dst, double const * src, hipStream_t s) override;
+  void update(hipStream_t s) override;
+
+  int n_levels() const { return (int)levels_.size(); }
+  // level l: 0 = coarsest
+  LevelOperator<Number> const & level_operator(int l) const { return *levels_[l].op; }
+  ChebyshevSmoother<Number> const & smoother(int l) const { return levels_[l].smoother; }
+  ChebyshevSmoother<Number> const & coarse_chebyshev() const { return coarse_cheb_; }
+  int                              last_coarse_iterations() const { return last_coarse_its_; }
+  // components for tests: transfer between level l-1 and l
+  void prolongate_and_add(int l, Number * dst_fine, Number const * src_coarse, hipStream_t s) const;
+  void restrict_and_add(int l, Number * dst_coarse, Number const * src_fine, hipStream_t s) const;
+
+private:
+  struct Level
+  {
+    std::unique_ptr<LevelOperator<Number>> op;
+    int                                    degree = 1;
+    bool                                   dg     = true;
+    DeviceVector<Number>                   defect, sol_a, sol_b, t;
+    Number *                               sol = nullptr, *sol_alt = nullptr;
+    ChebyshevSmoother<Number>              smoother;
+    DeviceBuffer<Number>                   P;  // p-transfer matrix to the next coarser level
+  };
+  void v_cycle(int l, hipStream_t s);
+  void coarse_solve(hipStream_t s);
+
+  std::shared_ptr<StructuredGrid const> grid_;
+  MultigridData                         data_;
+  std::vector<Level>                    levels_;
+  ChebyshevSmoother<Number>             coarse_cheb_;
+  DeviceVector<Number>                  coarse_inv_diag_, cr_;
+  PcgWorkspace<Number>                  coarse_ws_;
+  Reducer                               red_;
+  int                                   last_coarse_its_ = 0;
+};
+
+std::unique_ptr<PreconditionerBase> create_multigrid(std::shared_ptr<StructuredGrid const> grid,
+                                                     MultigridData const & data, double ip_factor,
+                                                     int precision, hipStream_t s);
+
+// Lanczos tridiagonal of CG (diag 1/alpha_k + beta_{k-1}/alpha_{k-1}, off sqrt(beta_k)/alpha_k):
+// smallest and largest eigenvalue (SolverCG's eigenvalue slot)
+void cg_tridiagonal_extreme_eigenvalues(std::vector<double> const & alpha,
+                                        std::vector<double> const & beta, double & lmin,
+                                        double & lmax);
+
+}  // namespace ExaDG

@@ -1,37 +1,338 @@
-// Streaming vector kernels (HBM-bound): diagonal inversion and precision conversion.
+// Streaming vector kernels (HBM-bound): reductions with wave-level shuffles, CG and
+// Chebyshev updates, diagonal inversion, precision conversion.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
 #include "laplace_operator.h"
+#include "vector_ops.h"
 
 namespace ExaDG
 {
 namespace
 {
-template<typename Number>
-__global__ void __launch_bounds__(256) invert_diagonal_kernel(Number * d, int64_t n)
+constexpr int kBlock = 256;
+
+inline int grid_for(int64_t n, int cap = 8192)
 {
-  int64_t const stride = (int64_t)gridDim.x * blockDim.x;
-  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+  int64_t const b = (n + kBlock - 1) / kBlock;
+  return (int)std::min<int64_t>(std::max<int64_t>(b, 1), cap);
+}
+
+// block-wide sum of one double per thread (wave64 shuffles, then LDS across waves)
+__device__ __forceinline__ double block_sum(double v)
+{
+  __shared__ double wsum[kBlock / 64];
+#pragma unroll
+  for(int off = 32; off > 0; off >>= 1)
+    v += __shfl_down(v, off, 64);
+  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if(lane == 0)
+    wsum[wave] = v;
+  __syncthreads();
+  double r = 0;
+  if(threadIdx.x == 0)
+    for(int w = 0; w < kBlock / 64; ++w)
+      r += wsum[w];
+  return r;  // valid on thread 0
+}
+
+template<typename Number, int K>
+__global__ void __launch_bounds__(kBlock)
+  reduce_stage1(Number const * a, Number const * b, int64_t n, double * partials)
+{
+  // K = 0: sum a*b ; K = 1: sum a*a and sum a*b (two partial arrays)
+  double        s0 = 0, s1 = 0;
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+  {
+    double const x = (double)a[i];
+    double const y = (double)b[i];
+    s0 += x * y;
+    if constexpr(K == 1)
+      s1 += x * x;
+  }
+  double const r0 = block_sum(s0);
+  __syncthreads();
+  if(threadIdx.x == 0)
+    partials[blockIdx.x] = r0;
+  if constexpr(K == 1)
+  {
+    double const r1 = block_sum(s1);
+    if(threadIdx.x == 0)
+      partials[Reducer::kMaxPartials + blockIdx.x] = r1;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+  reduce_stage2(double const * partials, int n_partials, double * out)
+{
+  double s = 0;
+  for(int i = threadIdx.x; i < n_partials; i += kBlock)
+    s += partials[i];
+  double const r = block_sum(s);
+  if(threadIdx.x == 0)
+    *out = r;
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock) set_zero_kernel(Number * x, int64_t n)
+{
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    x[i] = Number(0);
+}
+
+template<typename To, typename From>
+__global__ void __launch_bounds__(kBlock) convert_kernel(To * dst, From const * src, int64_t n)
+{
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    dst[i] = (To)src[i];
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock)
+  axpby_kernel(Number * y, Number a, Number const * x, Number b, int64_t n)
+{
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    y[i] = a * x[i] + b * y[i];
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock)
+  cg_update_kernel(Number * x, Number * r, Number const * p, Number const * v, double const * num,
+                   double const * den, int64_t n)
+{
+  Number const  alpha  = (Number)(*num / *den);
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+  {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * v[i];
+  }
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock)
+  cg_direction_kernel(Number * p, Number const * z, double const * num, double const * den,
+                      int64_t n)
+{
+  Number const  beta   = (Number)(*num / *den);
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    p[i] = z[i] + beta * p[i];
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock)
+  scale_kernel(Number * z, Number const * d, Number const * r, int64_t n)
+{
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    z[i] = d[i] * r[i];
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock)
+  chebyshev_kernel(Number * x_new, Number const * x, Number const * x_old, Number const * b,
+                   Number const * t, Number const * dinv, Number f1, Number f2, int variant,
+                   int64_t n)
+{
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+  {
+    if(variant == 0)
+      x_new[i] = f2 * dinv[i] * b[i];
+    else if(variant == 1)
+      x_new[i] = (Number(1) + f1) * x[i] + f2 * dinv[i] * (b[i] - t[i]);
+    else
+      x_new[i] = (Number(1) + f1) * x[i] - f1 * x_old[i] + f2 * dinv[i] * (b[i] - t[i]);
+  }
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock) shift_kernel(Number * x, double const * sum, int64_t n)
+{
+  Number const  mean   = (Number)(*sum / (double)n);
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    x[i] -= mean;
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock) sum_stage1(Number const * a, int64_t n, double * partials)
+{
+  double        s      = 0;
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    s += (double)a[i];
+  double const r = block_sum(s);
+  if(threadIdx.x == 0)
+    partials[blockIdx.x] = r;
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(kBlock) invert_diagonal_kernel(Number * d, int64_t n)
+{
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
   {
     Number const v = d[i];
     d[i]           = (fabs(v) > Number(1e-10)) ? Number(1) / v : Number(1);
   }
 }
-
-inline int grid_for(int64_t n)
-{
-  int64_t const b = (n + 255) / 256;
-  return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 8192);
-}
 }  // namespace
+
+Reducer::Reducer()
+{
+  partials_.resize(2 * kMaxPartials);
+  result_.resize(kSlots);
+  HIP_CHECK(hipHostMalloc((void **)&host_, kSlots * sizeof(double), hipHostMallocDefault));
+}
+
+Reducer::~Reducer()
+{
+  if(host_)
+    (void)hipHostFree(host_);
+}
+
+double const * Reducer::fetch(hipStream_t s, int n_slots)
+{
+  HIP_CHECK(hipMemcpyAsync(host_, result_.data(), n_slots * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return host_;
+}
+
+namespace vec
+{
+template<typename Number>
+void set_zero(Number * x, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL((set_zero_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, x, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void copy(Number * dst, Number const * src, int64_t n, hipStream_t s)
+{
+  HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(Number), hipMemcpyDeviceToDevice, s));
+}
+
+template<typename To, typename From>
+void convert(To * dst, From const * src, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL((convert_kernel<To, From>), dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void dot(Reducer & red, int slot, Number const * a, Number const * b, int64_t n, hipStream_t s)
+{
+  int const g = grid_for(n, Reducer::kMaxPartials);
+  hipLaunchKernelGGL((reduce_stage1<Number, 0>), dim3(g), dim3(kBlock), 0, s, a, b, n,
+                     red.partials());
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, red.partials(), g,
+                     red.result() + slot);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void norm2_and_dot(Reducer & red, int slot_aa, int slot_ab, Number const * a, Number const * b,
+                   int64_t n, hipStream_t s)
+{
+  int const g = grid_for(n, Reducer::kMaxPartials);
+  hipLaunchKernelGGL((reduce_stage1<Number, 1>), dim3(g), dim3(kBlock), 0, s, a, b, n,
+                     red.partials());
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, red.partials(), g,
+                     red.result() + slot_ab);
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s,
+                     red.partials() + Reducer::kMaxPartials, g, red.result() + slot_aa);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void axpby(Number * y, Number a, Number const * x, Number b, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL((axpby_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, y, a, x, b, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void cg_update(Number * x, Number * r, Number const * p, Number const * v, double const * num,
+               double const * den, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL((cg_update_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, x, r, p, v,
+                     num, den, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void cg_direction(Number * p, Number const * z, double const * num, double const * den, int64_t n,
+                  hipStream_t s)
+{
+  hipLaunchKernelGGL((cg_direction_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, p, z,
+                     num, den, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void scale(Number * z, Number const * d, Number const * r, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL((scale_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, z, d, r, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void chebyshev_update(Number * x_new, Number const * x, Number const * x_old, Number const * b,
+                      Number const * t, Number const * dinv, Number f1, Number f2, bool first,
+                      bool has_old, int64_t n, hipStream_t s)
+{
+  int const variant = first ? 0 : (has_old ? 2 : 1);
+  hipLaunchKernelGGL((chebyshev_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, x_new, x,
+                     x_old, b, t, dinv, f1, f2, variant, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
+void subtract_mean(Reducer & red, Number * x, int64_t n, hipStream_t s)
+{
+  int const g = grid_for(n, Reducer::kMaxPartials);
+  hipLaunchKernelGGL((sum_stage1<Number>), dim3(g), dim3(kBlock), 0, s, x, n, red.partials());
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, red.partials(), g,
+                     red.result() + 7);
+  hipLaunchKernelGGL((shift_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, x,
+                     red.result() + 7, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+#define EXADG_INST(T)                                                                            \
+  template void set_zero<T>(T *, int64_t, hipStream_t);                                          \
+  template void copy<T>(T *, T const *, int64_t, hipStream_t);                                   \
+  template void dot<T>(Reducer &, int, T const *, T const *, int64_t, hipStream_t);              \
+  template void norm2_and_dot<T>(Reducer &, int, int, T const *, T const *, int64_t, hipStream_t); \
+  template void axpby<T>(T *, T, T const *, T, int64_t, hipStream_t);                            \
+  template void cg_update<T>(T *, T *, T const *, T const *, double const *, double const *,     \
+                             int64_t, hipStream_t);                                              \
+  template void cg_direction<T>(T *, T const *, double const *, double const *, int64_t,         \
+                                hipStream_t);                                                    \
+  template void scale<T>(T *, T const *, T const *, int64_t, hipStream_t);                       \
+  template void chebyshev_update<T>(T *, T const *, T const *, T const *, T const *, T const *, T, \
+                                    T, bool, bool, int64_t, hipStream_t);                        \
+  template void subtract_mean<T>(Reducer &, T *, int64_t, hipStream_t);
+EXADG_INST(double)
+EXADG_INST(float)
+#undef EXADG_INST
+template void convert<float, double>(float *, double const *, int64_t, hipStream_t);
+template void convert<double, float>(double *, float const *, int64_t, hipStream_t);
+}  // namespace vec
 
 namespace Poisson
 {
 template<typename Number>
 void invert_diagonal_device(Number * d, int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL((invert_diagonal_kernel<Number>), dim3(grid_for(n)), dim3(256), 0, s, d, n);
+  hipLaunchKernelGGL((invert_diagonal_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, d, n);
   HIP_CHECK(hipGetLastError());
 }
 

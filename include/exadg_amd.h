@@ -107,6 +107,94 @@ int exadg_laplace_calculate_diagonal(exadg_laplace * op, void * diag, void * str
  * b is always FP64. (sine/application.h:32-90 with omega = 3 pi.) */
 int exadg_laplace_rhs_sine(exadg_laplace * op, double omega, double * b, void * stream);
 
+
+/* ---------------------------------------------------------------- preconditioners / solver */
+
+enum exadg_preconditioner_kind
+{
+  EXADG_PRECOND_NONE         = 0,
+  EXADG_PRECOND_POINT_JACOBI = 1,
+  EXADG_PRECOND_MULTIGRID    = 2
+};
+
+enum exadg_coarse_solver
+{
+  EXADG_COARSE_CG        = 0, /* MGCoarseKrylov: CG + point Jacobi to coarse_rel_tol       */
+  EXADG_COARSE_CHEBYSHEV = 1  /* MGCoarseChebyshev: degree chosen for coarse_rel_tol        */
+};
+
+/* MultigridData (solvers_and_preconditioners/multigrid/multigrid_parameters.h:39-313) for the
+ * hybrid p-multigrid DG_k -> ... -> DG_1 [-> continuous Q1]: the DG degrees are listed fine to
+ * coarse (dg_degrees[0] = operator degree, strictly decreasing; the last must be 1 when
+ * coarse_q1 = 1). precision is the V-cycle's floating-point type (MultigridNumber). */
+typedef struct exadg_mg_desc
+{
+  int32_t n_dg_levels;
+  int32_t dg_degrees[16];
+  int32_t coarse_q1;
+  int32_t precision;
+  int32_t smoother_degree;   /* Chebyshev degree, SmootherData::iterations (5)        */
+  double  smoothing_range;   /* SmootherData::smoothing_range (20)                    */
+  int32_t eig_cg_iterations; /* SmootherData::iterations_eigenvalue_estimation (20)   */
+  int32_t coarse_solver;     /* exadg_coarse_solver                                   */
+  double  coarse_rel_tol;    /* CoarseGridData solver_data.rel_tol (1e-3)             */
+  int32_t coarse_max_iter;
+  int32_t operator_is_singular;
+} exadg_mg_desc;
+
+/* Defaults of the reference for degree k: PSequenceType::Bisect (p -> max(p/2,1),
+ * multigrid_preconditioner_base.cpp:202-216), c-transfer to continuous Q1 after p-coarsening,
+ * FP32 V-cycle (multigrid_preconditioner_base.h:60), Chebyshev(5, 20, 20), coarse CG 1e-3. */
+int exadg_mg_desc_default(int degree, exadg_mg_desc * out);
+
+typedef struct exadg_preconditioner exadg_preconditioner;
+
+/* The preconditioner factory of Poisson::Operator::setup_preconditioner_and_solver
+ * (operator.cpp:306-398) for None / PointJacobi / Multigrid; `mg` is read for MULTIGRID only.
+ * The operator must be FP64 (the outer solver's precision). Setup includes the diagonals,
+ * the Chebyshev eigenvalue estimates and the coarse solver (PreconditionerBase::update). */
+int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_desc * mg,
+                                exadg_preconditioner ** out, void * stream);
+/* PreconditionerBase::vmult (preconditioner_base.h:33-67): dst = P^{-1} src, FP64 vectors. */
+int exadg_preconditioner_vmult(exadg_preconditioner * p, double * dst, const double * src,
+                               void * stream);
+/* PreconditionerBase::update: recompute diagonals / eigenvalue estimates. */
+int exadg_preconditioner_update(exadg_preconditioner * p, void * stream);
+int exadg_preconditioner_destroy(exadg_preconditioner * p);
+/* Multigrid diagnostics: number of levels and, per level (0 = coarsest), the CG estimate of
+ * lambda_max(D^{-1}A) used by the Chebyshev smoother (coarse level: the coarse solver's). */
+int exadg_preconditioner_mg_info(const exadg_preconditioner * p, int * n_levels,
+                                 double * lambda_max, int * chebyshev_degree,
+                                 int * last_coarse_iterations);
+
+/* SolverData (solvers_and_preconditioners/solvers/solver_data.h:91-146) */
+typedef struct exadg_solver_data
+{
+  int32_t max_iter;
+  double  abs_tol;
+  double  rel_tol;
+  int32_t rel_mode; /* 0: ||r|| <= max(abs, rel ||r0||) (ReductionControl); 1: rel ||b|| */
+} exadg_solver_data;
+
+typedef struct exadg_solve_info
+{
+  int32_t iterations;
+  int32_t converged;
+  int32_t not_finite;
+  double  r0;
+  double  r_final;
+  double  b_norm;
+  double  rho; /* average convergence rate (iterative_solvers_dealii_wrapper.h:63-78) */
+  double  n10; /* iterations per 10 orders of magnitude */
+} exadg_solve_info;
+
+/* Krylov::KrylovSolver::solve with CG (iterative_solvers_dealii_wrapper.h:136-221) on the FP64
+ * operator, preconditioned by `prec` (NULL = none); x holds the initial guess on entry.
+ * Returns EXADG_ERR_NOT_FINITE on NaN/Inf and EXADG_ERR_NO_CONVERGE when max_iter is hit
+ * (info is filled in both cases). */
+int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
+                   const exadg_solver_data * data, exadg_solve_info * info, void * stream);
+
 #ifdef __cplusplus
 }
 #endif

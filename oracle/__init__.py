@@ -79,6 +79,24 @@ def lib():
         L.oracle_cg_solve.restype = ctypes.c_int
         L.oracle_cg_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, dp]
+        L.oracle_mg_create.restype = ctypes.c_void_p
+        L.oracle_mg_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_double]
+        L.oracle_mg_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_mg_vmult.argtypes = [ctypes.c_void_p, dp, dp]
+        L.oracle_mg_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.oracle_mg_level_vmult.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+        L.oracle_mg_prolongate_add.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+        L.oracle_mg_restrict_add.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+        L.oracle_mg_smoother_vmult.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]
+        L.oracle_cg_solve_mg.restype = ctypes.c_int
+        L.oracle_cg_solve_mg.argtypes = [ctypes.c_void_p, ctypes.c_void_p, dp, dp, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
         L.oracle_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -173,3 +191,90 @@ class LaplaceOperator:
             raise RuntimeError(lib().oracle_last_error().decode())
         return x, dict(iterations=int(info[0]), r0=info[1], r_final=info[2],
                        converged=bool(info[3]))
+
+
+def bisect_degrees(k):
+    """PSequenceType::Bisect: k, k/2, ..., 1 (multigrid_preconditioner_base.cpp:202-216)."""
+    out = [k]
+    while out[-1] > 1:
+        out.append(max(out[-1] // 2, 1))
+    return out
+
+
+class Multigrid:
+    """Oracle hybrid p-multigrid V-cycle (FP64)."""
+
+    def __init__(self, mesh, degrees, coarse_q1=True, smoother_degree=5, smoothing_range=20.0,
+                 eig_its=20, coarse_solver=0, coarse_rel_tol=1e-3, coarse_max_iter=10000,
+                 singular=False, ip_factor=1.0):
+        arr = (ctypes.c_int * len(degrees))(*degrees)
+        self._h = lib().oracle_mg_create(ctypes.byref(mesh), arr, len(degrees), int(coarse_q1),
+                                         smoother_degree, smoothing_range, eig_its, coarse_solver,
+                                         coarse_rel_tol, coarse_max_iter, int(singular), ip_factor)
+        if not self._h:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        self.info()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_mg_destroy(self._h)
+            self._h = None
+
+    def info(self):
+        nl = ctypes.c_int()
+        n = (ctypes.c_longlong * 32)()
+        deg, dg, cd = (ctypes.c_int * 32)(), (ctypes.c_int * 32)(), (ctypes.c_int * 32)()
+        lmax = np.zeros(32)
+        cits = ctypes.c_int()
+        lib().oracle_mg_info(self._h, ctypes.byref(nl), n, deg, dg, _p(lmax), cd, ctypes.byref(cits))
+        L = nl.value
+        self.n_levels = L
+        self.sizes = [n[i] for i in range(L)]
+        self.degrees = [deg[i] for i in range(L)]
+        self.dg = [bool(dg[i]) for i in range(L)]
+        self.lambda_max = lmax[:L].copy()
+        self.cheb_degree = [cd[i] for i in range(L)]
+        self.last_coarse_iterations = cits.value
+        return self
+
+    def vmult(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.empty(self.sizes[-1])
+        lib().oracle_mg_vmult(self._h, _p(dst), _p(src))
+        return dst
+
+    def smoother_vmult(self, level, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.empty(self.sizes[level])
+        td = np.zeros(2)
+        lib().oracle_mg_smoother_vmult(self._h, level, _p(dst), _p(src), _p(td))
+        return dst, td[0], td[1]
+
+    def level_vmult(self, level, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.empty(self.sizes[level])
+        lib().oracle_mg_level_vmult(self._h, level, _p(dst), _p(src))
+        return dst
+
+    def prolongate(self, level, coarse):
+        fine = np.zeros(self.sizes[level])
+        lib().oracle_mg_prolongate_add(self._h, level, _p(fine),
+                                       _p(np.ascontiguousarray(coarse, dtype=np.float64)))
+        return fine
+
+    def restrict(self, level, fine):
+        coarse = np.zeros(self.sizes[level - 1])
+        lib().oracle_mg_restrict_add(self._h, level, _p(coarse),
+                                     _p(np.ascontiguousarray(fine, dtype=np.float64)))
+        return coarse
+
+
+def cg_solve_mg(op, mg, b, x0=None, rel_tol=1e-10, abs_tol=1e-20, max_iter=10000, rel_mode=0):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(op.n) if x0 is None else np.array(x0, dtype=np.float64)
+    info = np.zeros(4)
+    rc = lib().oracle_cg_solve_mg(op._h, mg._h if mg is not None else None, _p(b), _p(x), rel_tol,
+                                  abs_tol, max_iter, rel_mode, _p(info))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return x, dict(iterations=int(info[0]), r0=info[1], r_final=info[2], converged=bool(info[3]))

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "oracle_core.h"
+#include "oracle_mg.h"
 #include "oracle_solver.h"
 
 using namespace oracle;
@@ -116,6 +117,122 @@ int oracle_cg_solve(void * h, int use_jacobi, double const * b, double * x, doub
   {
     JacobiCG solver(*op, use_jacobi != 0);
     SolveInfo si = solver.solve(b, x, rel_tol, abs_tol, max_iter);
+    info[0]      = si.iterations;
+    info[1]      = si.r0;
+    info[2]      = si.r_final;
+    info[3]      = si.converged ? 1.0 : 0.0;
+    return 0;
+  }
+  catch(std::exception const & e)
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// hybrid p-multigrid (DG degrees fine to coarse, optional continuous Q1 level)
+void * oracle_mg_create(oracle_mesh const * m, int const * degrees, int n_degrees, int coarse_q1,
+                        int smoother_degree, double smoothing_range, int eig_its,
+                        int coarse_solver, double coarse_rel_tol, int coarse_max_iter,
+                        int singular, double ip_factor)
+{
+  try
+  {
+    MeshDesc d;
+    std::memcpy(&d, m, sizeof(MeshDesc));
+    MGDesc md;
+    md.degrees.assign(degrees, degrees + n_degrees);
+    md.coarse_q1       = coarse_q1 != 0;
+    md.smoother_degree = smoother_degree;
+    md.smoothing_range = smoothing_range;
+    md.eig_its         = eig_its;
+    md.coarse_solver   = coarse_solver;
+    md.coarse_rel_tol  = coarse_rel_tol;
+    md.coarse_max_iter = coarse_max_iter;
+    md.singular        = singular != 0;
+    return new Multigrid(d, md, ip_factor);
+  }
+  catch(std::exception const & e)
+  {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void oracle_mg_destroy(void * h) { delete static_cast<Multigrid *>(h); }
+
+void oracle_mg_vmult(void * h, double * dst, double const * src)
+{
+  static_cast<Multigrid *>(h)->vmult(dst, src);
+}
+
+// per level (0 = coarsest): n, degree, dg flag, lambda_max estimate, Chebyshev degree
+void oracle_mg_info(void * h, int * n_levels, long long * n, int * degree, int * dg, double * lmax,
+                    int * cheb_degree, int * coarse_its)
+{
+  auto * mg = static_cast<Multigrid *>(h);
+  *n_levels = (int)mg->levels.size();
+  for(size_t l = 0; l < mg->levels.size(); ++l)
+  {
+    auto const & L   = mg->levels[l];
+    n[l]             = L.n;
+    degree[l]        = L.degree;
+    dg[l]            = L.dg ? 1 : 0;
+    Chebyshev const & c = l == 0 ? mg->coarse_cheb : L.smoother;
+    lmax[l]          = c.lambda_max;
+    cheb_degree[l]   = c.degree;
+  }
+  *coarse_its = mg->last_coarse_its;
+}
+
+void oracle_mg_level_vmult(void * h, int l, double * dst, double const * src)
+{
+  static_cast<Multigrid *>(h)->levels[l].A(dst, src);
+}
+
+void oracle_mg_prolongate_add(void * h, int l, double * fine, double const * coarse)
+{
+  static_cast<Multigrid *>(h)->prolongate_add(l, fine, coarse);
+}
+
+void oracle_mg_restrict_add(void * h, int l, double * coarse, double const * fine)
+{
+  static_cast<Multigrid *>(h)->restrict_add(l, coarse, fine);
+}
+
+// Chebyshev smoother of level l (l >= 1): dst = S src (vmult, zero initial guess); theta/delta
+void oracle_mg_smoother_vmult(void * h, int l, double * dst, double const * src, double * theta_delta)
+{
+  auto const & L = static_cast<Multigrid *>(h)->levels[l];
+  std::vector<double> b(src, src + L.n), x(L.n, 0.0);
+  L.smoother.vmult(x, b);
+  std::copy(x.begin(), x.end(), dst);
+  theta_delta[0] = L.smoother.theta;
+  theta_delta[1] = L.smoother.delta;
+}
+
+// PCG on the SIPG operator with the multigrid preconditioner; rel_mode 0: vs ||r0||, 1: vs ||b||
+int oracle_cg_solve_mg(void * h_op, void * h_mg, double const * b, double * x, double rel_tol,
+                       double abs_tol, int max_iter, int rel_mode, double * info)
+{
+  auto * op = static_cast<LaplaceOperator *>(h_op);
+  auto * mg = static_cast<Multigrid *>(h_mg);
+  try
+  {
+    int64_t const n = op->n_dofs();
+    Apply         A = [&](double * dst, double const * src) { op->vmult(dst, src); };
+    Apply         P = [&](double * dst, double const * src) { mg->vmult(dst, src); };
+    double        tol_rel = rel_tol;
+    if(rel_mode == 1)
+    {
+      // relative to ||b||: convert to an absolute tolerance
+      double bn = 0;
+      for(int64_t i = 0; i < n; ++i)
+        bn += b[i] * b[i];
+      abs_tol = std::max(abs_tol, rel_tol * std::sqrt(bn));
+      tol_rel = 0.0;
+    }
+    SolveInfo si = pcg(n, A, mg ? &P : nullptr, b, x, tol_rel, abs_tol, max_iter);
     info[0]      = si.iterations;
     info[1]      = si.r0;
     info[2]      = si.r_final;
