@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--degree", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solve", action="store_true",
+                    help="also time the CG + p-multigrid solve (config 1 and curved k=5 48^3)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run plain device copies of the same vector (HBM counter calibration)")
     args = ap.parse_args()
@@ -124,6 +126,18 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
     }
+    if args.solve:
+        from exadg_amd import solve_bench
+        del x, y
+        torch.cuda.empty_cache()
+        sol = {}
+        sol["config1_cart_k3_8c"] = solve_bench.compare_arms(
+            exadg_amd, torch, exadg_amd.mesh_desc(8), exadg_amd.bc_desc(), 3, [3, 1], 3.141592653589793,
+            exadg_amd.COARSE_CG, reps=3)
+        sol["config3_curved_k5_48c"] = solve_bench.compare_arms(
+            exadg_amd, torch, exadg_amd.mesh_desc(48, deformation=0.1, mapping_degree=5),
+            exadg_amd.bc_desc(), 5, [5, 3, 2, 1], 3.141592653589793, exadg_amd.COARSE_CHEBYSHEV)
+        out["solve"] = sol
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.degree, 32)
     print(json.dumps(out), flush=True)
