@@ -13,6 +13,8 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -45,6 +47,53 @@ def cpu_baseline(degree, n_cells_1d, target_s=10.0):
                       f"{len(times)} calls"}
 
 
+def run_guess(ex, torch, args):
+    """Config 4: pressure sequence on a periodic-x/y walled-z box; train on the first steps,
+    A/B on the rest. FP32-MG arms: zero / extrapolated / learned guess; FP64-MG arms (the plain
+    method): zero / extrapolated guess."""
+    from exadg_amd import pressure_sequence as ps
+
+    def log(msg):
+        print(f"[guess] {msg}", file=sys.stderr, flush=True)
+
+    k, nc = args.guess_degree, args.guess_cells
+    op, prec32 = ps.setup_box(ex, nc, k, ex.FP32)
+    _, prec64 = ps.setup_box(ex, nc, k, ex.FP64)
+    modes = ex.fourier_modes(2026, 1, args.guess_kmax)
+    log(f"{op.n} DoF, {len(modes)} Fourier modes; generating {args.guess_steps} steps")
+    t0 = time.perf_counter()
+    P, Bv, its, times = ps.generate(ex, torch, op, prec32, modes, args.guess_steps, log=log)
+    t_gen = time.perf_counter() - t0
+    net = ex.GuessNet(k, history=3, width=128, seed=1)
+    log("training")
+    losses, t_train = ps.train(ex, net, torch, op, P, Bv, args.guess_train, args.guess_epochs,
+                               lr=args.guess_lr, loss=ex.GUESS_LOSS_RESIDUAL)
+    held = net.evaluate(P, Bv, args.guess_train - 3, args.guess_steps, op=op,
+                        loss=ex.GUESS_LOSS_RESIDUAL)
+    log(f"train loss {losses[0]:.4g} -> {losses[-1]:.4g}, held-out {held:.4g}; A/B FP32 MG")
+    ab32 = ps.ab_test(ex, torch, op, prec32, net, P, Bv, args.guess_train, args.guess_steps)
+    log("A/B FP64 MG")
+    ab64 = ps.ab_test(ex, torch, op, prec64, None, P, Bv, args.guess_train, args.guess_steps,
+                      arms=("zero", "extrapolation"))
+    plain = ab64["extrapolation"]["total_time_s"]
+    return {
+        "workload": f"pressure sequence, [0,1]^3 periodic x/y, Neumann walls z, {nc}^3 cells, "
+                    f"degree {k}, {op.n} DoF, dt 0.01, {len(modes)} modes |n| <= {args.guess_kmax}",
+        "steps": args.guess_steps, "train_steps": args.guess_train,
+        "generation": {"time_s": round(t_gen, 3), "mean_iterations": float(np.mean(its[3:]))},
+        "training": {"epochs": args.guess_epochs, "time_s": round(t_train, 3),
+                     "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1],
+                     "held_out_loss": held,
+                     "loss": "residual |A e|^2 of the guess per step, divided by the "
+                             "extrapolation's: 1 = no better than the extrapolation"},
+        "fp32_mg": ab32, "fp64_mg": ab64,
+        "speedup_learned_fp32mg_vs_plain_fp64mg_extrapolated": round(
+            plain / ab32["learned"]["total_time_s"], 3),
+        "speedup_learned_fp32mg_vs_fp64mg_zero_guess": round(
+            ab64["zero"]["total_time_s"] / ab32["learned"]["total_time_s"], 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +104,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true",
                     help="also time the CG + p-multigrid solve (config 1 and curved k=5 48^3)")
+    ap.add_argument("--guess", action="store_true",
+                    help="also run the learned-initial-guess pressure sequence (config 4)")
+    ap.add_argument("--guess-cells", type=int, default=32)
+    ap.add_argument("--guess-degree", type=int, default=4)
+    ap.add_argument("--guess-steps", type=int, default=400)
+    ap.add_argument("--guess-train", type=int, default=300)
+    ap.add_argument("--guess-epochs", type=int, default=20)
+    ap.add_argument("--guess-kmax", type=int, default=8)
+    ap.add_argument("--guess-lr", type=float, default=1e-3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run plain device copies of the same vector (HBM counter calibration)")
     args = ap.parse_args()
@@ -138,6 +196,10 @@ def main():
             exadg_amd, torch, exadg_amd.mesh_desc(48, deformation=0.1, mapping_degree=5),
             exadg_amd.bc_desc(), 5, [5, 3, 2, 1], 3.141592653589793, exadg_amd.COARSE_CHEBYSHEV)
         out["solve"] = sol
+    if args.guess:
+        del x, y
+        torch.cuda.empty_cache()
+        out["learned_guess"] = run_guess(exadg_amd, torch, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.degree, 32)
     print(json.dumps(out), flush=True)

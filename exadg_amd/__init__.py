@@ -15,6 +15,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "exadg_amd.h")
 EXADG_OK = 0
 FP64, FP32 = 0, 1
 DIRICHLET, NEUMANN = 0, 1
+GUESS_LOSS_L2, GUESS_LOSS_ENERGY, GUESS_LOSS_RESIDUAL = 0, 1, 2
 
 _lib = None
 
@@ -123,6 +124,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
         ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
         L.exadg_last_error.restype = ctypes.c_char_p
         L.exadg_last_error.argtypes = []
         sig = {
@@ -145,6 +147,25 @@ def lib():
             "exadg_preconditioner_mg_info": [vp, ip, ctypes.POINTER(ctypes.c_double), ip, ip],
             "exadg_cg_solve": [vp, vp, vp, vp, ctypes.POINTER(SolverData), ctypes.POINTER(SolveInfo),
                                vp],
+            "exadg_fourier_modes": [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ip],
+            "exadg_laplace_rhs_divergence_fourier": [vp, dp, ctypes.c_int, ctypes.c_double, vp, vp],
+            "exadg_vector_subtract_mean": [vp, ctypes.c_int64, vp],
+            "exadg_extrapolate": [ctypes.c_int, ctypes.POINTER(vp), vp, ctypes.c_int64, vp],
+            "exadg_guess_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                   ctypes.POINTER(vp)],
+            "exadg_guess_destroy": [vp],
+            "exadg_guess_n_params": [vp, ip, ip, ip],
+            "exadg_guess_get_params": [vp, vp, vp],
+            "exadg_guess_set_params": [vp, vp, vp],
+            "exadg_guess_set_scales": [vp, dp, ctypes.c_int],
+            "exadg_guess_get_scales": [vp, dp, ctypes.c_int],
+            "exadg_guess_loss_and_grad": [vp, vp, vp, ctypes.c_int, vp, dp, vp],
+            "exadg_guess_train": [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int64, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_uint64, dp, vp],
+            "exadg_guess_evaluate": [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int64, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, dp, vp],
+            "exadg_guess_predict": [vp, ctypes.POINTER(vp), vp, vp, ctypes.c_int64, ip, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -232,6 +253,18 @@ class LaplaceOperator:
         assert b.is_cuda and b.dtype == torch.float64 and b.numel() == self.n
         check(lib().exadg_laplace_rhs_sine(self._h, omega, _ptr(b), _stream(stream)))
 
+    def rhs_divergence_fourier(self, modes, t, b, stream=None):
+        """Weak DG divergence (central flux, no-slip walls) of the Fourier field `modes`
+        (numpy float64, 10 per mode) at time t; mean removed for a singular operator."""
+        import numpy as np
+        import torch
+        assert b.is_cuda and b.dtype == torch.float64 and b.numel() == self.n
+        m = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1)
+        assert m.size % 10 == 0 and m.size > 0
+        check(lib().exadg_laplace_rhs_divergence_fourier(
+            self._h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m.size // 10, t, _ptr(b),
+            _stream(stream)))
+
 
 class Preconditioner:
     """PreconditionerBase: None / PointJacobi / Multigrid (FP32 or FP64 V-cycle)."""
@@ -290,3 +323,137 @@ def cg_solve(op, b, x, prec=None, rel_tol=1e-10, abs_tol=1e-20, max_iter=10000, 
     if rc != EXADG_OK and raise_on_error:
         raise ExaDGError(rc, lib().exadg_last_error().decode())
     return out
+
+
+def fourier_modes(seed, k_min=1, k_max=8):
+    """Seeded solenoidal Fourier field over the shell k_min <= |n| <= k_max (numpy, shape
+    (n_modes, 10)): n(3), d(3), A, phi, omega, 0 per mode."""
+    import numpy as np
+    cnt = ctypes.c_int()
+    check(lib().exadg_fourier_modes(seed, k_min, k_max, 0, None, ctypes.byref(cnt)))
+    out = np.zeros((cnt.value, 10), dtype=np.float64)
+    check(lib().exadg_fourier_modes(seed, k_min, k_max, cnt.value,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    ctypes.byref(cnt)))
+    return out
+
+
+def subtract_mean(v, stream=None):
+    import torch
+    assert v.is_cuda and v.dtype == torch.float64 and v.is_contiguous()
+    check(lib().exadg_vector_subtract_mean(_ptr(v), v.numel(), _stream(stream)))
+
+
+def _history(vectors):
+    arr = (ctypes.c_void_p * max(1, len(vectors)))()
+    for i, t in enumerate(vectors):
+        arr[i] = t.data_ptr()
+    return arr
+
+
+def extrapolate(history, x0, stream=None):
+    """x0 = sum_i beta_i history[i] (history newest first, FP64 device vectors)."""
+    for t in history:
+        assert t.is_cuda and t.numel() == x0.numel() and t.dtype == x0.dtype
+    check(lib().exadg_extrapolate(len(history), _history(history), _ptr(x0), x0.numel(),
+                                  _stream(stream)))
+
+
+class GuessNet:
+    """Learned initial guess: per-cell MLP correcting the order-`history` extrapolation."""
+
+    def __init__(self, degree, history=3, width=128, seed=1):
+        self._h = ctypes.c_void_p()
+        check(lib().exadg_guess_create(degree, history, width, seed, ctypes.byref(self._h)))
+        npar, nf, no = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().exadg_guess_n_params(self._h, ctypes.byref(npar), ctypes.byref(nf),
+                                         ctypes.byref(no)))
+        self.degree, self.history, self.width = degree, history, width
+        self.n_params, self.n_features, self.n_outputs = npar.value, nf.value, no.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().exadg_guess_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        """Copy (FP32, device) of the flat parameters [W1|b1|W2|b2|W3|b3]."""
+        import torch
+        out = torch.empty(self.n_params, dtype=torch.float32, device="cuda")
+        check(lib().exadg_guess_get_params(self._h, _ptr(out), _stream(None)))
+        return out
+
+    def set_params(self, values):
+        import torch
+        v = values.to(device="cuda", dtype=torch.float32).contiguous()
+        assert v.numel() == self.n_params
+        torch.cuda.current_stream().synchronize()
+        check(lib().exadg_guess_set_params(self._h, _ptr(v), _stream(None)))
+
+    def scales(self):
+        """[s_0 .. s_{H-1}, s_b, s_t]: RMS of the history differences, the rhs, the targets."""
+        n = self.history + 2
+        arr = (ctypes.c_double * n)()
+        check(lib().exadg_guess_get_scales(self._h, arr, n))
+        return list(arr)
+
+    def set_scales(self, scales):
+        n = self.history + 2
+        assert len(scales) == n
+        arr = (ctypes.c_double * n)(*scales)
+        check(lib().exadg_guess_set_scales(self._h, arr, n))
+
+    def loss_and_grad(self, X, T, stream=None):
+        import torch
+        B = X.shape[0]
+        assert X.dtype == torch.float32 and X.is_contiguous() and X.shape[1] == self.n_features
+        assert T.dtype == torch.float32 and T.is_contiguous() and tuple(T.shape) == (B, self.n_outputs)
+        g = torch.empty(self.n_params, dtype=torch.float32, device=X.device)
+        loss = ctypes.c_double()
+        check(lib().exadg_guess_loss_and_grad(self._h, _ptr(X), _ptr(T), B, _ptr(g),
+                                              ctypes.byref(loss), _stream(stream)))
+        torch.cuda.synchronize()
+        return loss.value, g
+
+    def train(self, P, Bv, t_begin, t_end, epochs, op=None, loss=None, batch_cells=4096, lr=1e-3,
+              seed=7, stream=None):
+        """Adam on the FP64 sequence P, Bv (steps x n_dofs, device). loss: GUESS_LOSS_* (default
+        ENERGY with an operator, else L2). Returns the last epoch's normalized loss."""
+        import torch
+        assert P.dtype == torch.float64 and Bv.dtype == torch.float64 and P.shape == Bv.shape
+        assert P.is_contiguous() and Bv.is_contiguous()
+        if loss is None:
+            loss = GUESS_LOSS_ENERGY if op is not None else GUESS_LOSS_L2
+        out = ctypes.c_double()
+        check(lib().exadg_guess_train(self._h, op.handle if op is not None else None, loss, _ptr(P),
+                                      _ptr(Bv), P.shape[1], t_begin, t_end, epochs, batch_cells, lr,
+                                      seed, ctypes.byref(out), _stream(stream)))
+        return out.value
+
+    def evaluate(self, P, Bv, t_begin, t_end, op=None, loss=None, batch_cells=4096, stream=None):
+        import torch
+        assert P.dtype == torch.float64 and Bv.dtype == torch.float64 and P.shape == Bv.shape
+        if loss is None:
+            loss = GUESS_LOSS_ENERGY if op is not None else GUESS_LOSS_L2
+        out = ctypes.c_double()
+        check(lib().exadg_guess_evaluate(self._h, op.handle if op is not None else None, loss,
+                                         _ptr(P), _ptr(Bv), P.shape[1], t_begin, t_end, batch_cells,
+                                         ctypes.byref(out), _stream(stream)))
+        return out.value
+
+    def predict(self, history, rhs, x0, stream=None):
+        """x0 from the FP64 history (newest first) and rhs; returns True if the network output was
+        non-finite and the plain extrapolation was used."""
+        assert len(history) == self.history
+        n3 = self.n_outputs
+        assert x0.numel() % n3 == 0
+        fb = ctypes.c_int()
+        check(lib().exadg_guess_predict(self._h, _history(history), _ptr(rhs), _ptr(x0),
+                                        x0.numel() // n3, ctypes.byref(fb), _stream(stream)))
+        return bool(fb.value)

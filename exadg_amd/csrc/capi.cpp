@@ -1,11 +1,14 @@
 // extern "C" boundary: every entry point catches, records the message and returns a status.
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
 
 #include "../../include/exadg_amd.h"
 #include "common.h"
+#include "guess.h"
 #include "krylov.h"
 #include "laplace_operator.h"
 #include "multigrid.h"
@@ -57,6 +60,11 @@ struct exadg_laplace
 {
   std::unique_ptr<ExaDG::Poisson::LaplaceOperatorBase> op;
   std::unique_ptr<ExaDG::Krylov::SolverCG>             cg;
+};
+
+struct exadg_guess
+{
+  std::unique_ptr<ExaDG::GuessNet> net;
 };
 
 struct exadg_preconditioner
@@ -289,6 +297,232 @@ int exadg_preconditioner_mg_info(const exadg_preconditioner * p, int * n_levels,
       mg_info<float>(p->p.get(), n_levels, lmax, cheb_degree, coarse_its);
     else
       mg_info<double>(p->p.get(), n_levels, lmax, cheb_degree, coarse_its);
+  });
+}
+
+int exadg_fourier_modes(uint64_t seed, int k_min, int k_max, int max_modes, double * modes,
+                        int * n_modes)
+{
+  return guarded([&] {
+    require(n_modes && k_min >= 1 && k_max >= k_min && k_max <= 16, "invalid Fourier shell");
+    // the shell k_min <= |n| <= k_max, one vector of each pair +-n (first nonzero component
+    // positive), in lexicographic order
+    std::vector<std::array<int, 3>> shell;
+    for(int a = -k_max; a <= k_max; ++a)
+      for(int b = -k_max; b <= k_max; ++b)
+        for(int c = -k_max; c <= k_max; ++c)
+        {
+          int const r2 = a * a + b * b + c * c;
+          if(r2 < k_min * k_min || r2 > k_max * k_max)
+            continue;
+          if(a < 0 || (a == 0 && (b < 0 || (b == 0 && c < 0))))
+            continue;
+          shell.push_back({a, b, c});
+        }
+    *n_modes = (int)shell.size();
+    if(modes == nullptr)
+      return;
+    require(max_modes >= (int)shell.size(), "mode buffer too small");
+    uint64_t s    = seed;
+    auto     next = [&s]() {
+      uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+      z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for(size_t m = 0; m < shell.size(); ++m)
+    {
+      int const *  n  = shell[m].data();
+      double const kn = std::sqrt((double)(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
+      // xi uniform on the sphere; d = (n/|n|) x xi, so div u = 0
+      double const z = 2.0 * next() - 1.0, az = 2.0 * M_PI * next(), r = std::sqrt(1.0 - z * z);
+      double const xi[3] = {r * std::cos(az), r * std::sin(az), z};
+      double const kh[3] = {n[0] / kn, n[1] / kn, n[2] / kn};
+      double *     md    = modes + 10 * m;
+      md[0] = n[0], md[1] = n[1], md[2] = n[2];
+      md[3] = kh[1] * xi[2] - kh[2] * xi[1];
+      md[4] = kh[2] * xi[0] - kh[0] * xi[2];
+      md[5] = kh[0] * xi[1] - kh[1] * xi[0];
+      md[6] = std::pow(kn, -5.0 / 3.0);
+      md[7] = 2.0 * M_PI * next();
+      md[8] = kn;
+      md[9] = 0.0;
+    }
+  });
+}
+
+int exadg_laplace_rhs_divergence_fourier(exadg_laplace * op, const double * modes, int n_modes,
+                                         double t, double * b, void * stream)
+{
+  return guarded([&] {
+    require(op, "null operator");
+    op->op->rhs_divergence_fourier(modes, n_modes, t, b, stream_of(stream));
+  });
+}
+
+int exadg_vector_subtract_mean(double * v, int64_t n, void * stream)
+{
+  return guarded([&] {
+    require(v && n > 0, "invalid vector");
+    ExaDG::Reducer red;
+    ExaDG::vec::subtract_mean<double>(red, v, n, stream_of(stream));
+    HIP_CHECK(hipStreamSynchronize(stream_of(stream)));
+  });
+}
+
+int exadg_extrapolate(int order, const double * const * history, double * x0, int64_t n, void * stream)
+{
+  return guarded([&] {
+    require(history && x0 && n > 0, "invalid argument");
+    double beta[4];
+    ExaDG::extrapolation_beta(order, beta);
+    ExaDG::History<double> h;
+    for(int k = 0; k < order; ++k)
+    {
+      require(history[k] != nullptr, "null history vector");
+      h.p[k] = history[k];
+    }
+    hipError_t e = ExaDG::launch_combine(x0, h, order, beta, nullptr, 0.0, nullptr, n, nullptr,
+                                         stream_of(stream));
+    if(e != hipSuccess)
+      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  });
+}
+
+int exadg_guess_create(int degree, int history, int width, uint64_t seed, exadg_guess ** out)
+{
+  return guarded([&] {
+    require(out != nullptr, "null argument");
+    auto g = std::make_unique<exadg_guess>();
+    g->net = std::make_unique<ExaDG::GuessNet>(degree, history, width, seed);
+    *out   = g.release();
+  });
+}
+
+int exadg_guess_destroy(exadg_guess * g)
+{
+  return guarded([&] { delete g; });
+}
+
+int exadg_guess_n_params(const exadg_guess * g, int * n_params, int * n_features, int * n_outputs)
+{
+  return guarded([&] {
+    require(g, "null argument");
+    if(n_params)
+      *n_params = g->net->n_params();
+    if(n_features)
+      *n_features = g->net->n_features();
+    if(n_outputs)
+      *n_outputs = g->net->n_cell_dofs();
+  });
+}
+
+int exadg_guess_get_params(const exadg_guess * g, float * dst, void * stream)
+{
+  return guarded([&] {
+    require(g && dst, "null argument");
+    hipStream_t const s = stream_of(stream);
+    HIP_CHECK(hipMemcpyAsync(dst, g->net->params(), sizeof(float) * g->net->n_params(),
+                             hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int exadg_guess_set_params(exadg_guess * g, const float * src, void * stream)
+{
+  return guarded([&] {
+    require(g && src, "null argument");
+    hipStream_t const s = stream_of(stream);
+    HIP_CHECK(hipMemcpyAsync(g->net->params(), src, sizeof(float) * g->net->n_params(),
+                             hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int exadg_guess_set_scales(exadg_guess * g, const double * scales, int n)
+{
+  return guarded([&] {
+    require(g && scales && n == g->net->history() + 2, "expected history + 2 scales");
+    g->net->set_scales(std::vector<double>(scales, scales + n));
+  });
+}
+
+int exadg_guess_get_scales(const exadg_guess * g, double * scales, int n)
+{
+  return guarded([&] {
+    require(g && scales && n == g->net->history() + 2, "expected history + 2 scales");
+    std::vector<double> const v = g->net->scales();
+    std::copy(v.begin(), v.end(), scales);
+  });
+}
+
+int exadg_guess_loss_and_grad(exadg_guess * g, const float * X, const float * T, int B, float * grad,
+                              double * loss, void * stream)
+{
+  return guarded([&] {
+    require(g && X && T && B > 0, "invalid argument");
+    hipStream_t const s = stream_of(stream);
+    double const      l = g->net->loss_and_grad(X, T, B, s);
+    if(loss)
+      *loss = l;
+    if(grad)
+      HIP_CHECK(hipMemcpyAsync(grad, g->net->grads(), sizeof(float) * g->net->n_params(),
+                               hipMemcpyDeviceToDevice, s));
+  });
+}
+
+namespace
+{
+ExaDG::GuessLoss guess_loss(int kind)
+{
+  if(kind < EXADG_GUESS_LOSS_L2 || kind > EXADG_GUESS_LOSS_RESIDUAL)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "unknown guess loss");
+  return static_cast<ExaDG::GuessLoss>(kind);
+}
+}  // namespace
+
+int exadg_guess_train(exadg_guess * g, exadg_laplace * op, int loss_kind, const double * P,
+                      const double * Bv, int64_t n_dofs, int t_begin, int t_end, int epochs,
+                      int batch_cells, double lr, uint64_t seed, double * loss, void * stream)
+{
+  return guarded([&] {
+    require(g && P && Bv && n_dofs > 0 && epochs >= 0 && batch_cells > 0 && lr > 0,
+            "invalid argument");
+    ExaDG::GuessLoss const kind = guess_loss(loss_kind);
+    require(kind == ExaDG::GuessLoss::l2 || op != nullptr, "this loss needs the operator");
+    double const l = g->net->train(op ? op->op.get() : nullptr, kind, P, Bv, n_dofs, t_begin, t_end,
+                                   epochs, batch_cells, (float)lr, seed, stream_of(stream));
+    if(loss)
+      *loss = l;
+  });
+}
+
+int exadg_guess_evaluate(exadg_guess * g, exadg_laplace * op, int loss_kind, const double * P,
+                         const double * Bv, int64_t n_dofs, int t_begin, int t_end, int batch_cells,
+                         double * loss, void * stream)
+{
+  return guarded([&] {
+    require(g && P && Bv && n_dofs > 0 && batch_cells > 0, "invalid argument");
+    ExaDG::GuessLoss const kind = guess_loss(loss_kind);
+    require(kind == ExaDG::GuessLoss::l2 || op != nullptr, "this loss needs the operator");
+    double const l = g->net->evaluate(op ? op->op.get() : nullptr, kind, P, Bv, n_dofs, t_begin,
+                                      t_end, batch_cells, stream_of(stream));
+    if(loss)
+      *loss = l;
+  });
+}
+
+int exadg_guess_predict(exadg_guess * g, const double * const * history, const double * rhs,
+                        double * x0, int64_t n_cells, int * used_fallback, void * stream)
+{
+  return guarded([&] {
+    require(g && history && rhs && x0 && n_cells > 0, "invalid argument");
+    ExaDG::History<double> h;
+    for(int k = 0; k < g->net->history(); ++k)
+      h.p[k] = history[k];
+    int const bad = g->net->predict(x0, h, rhs, n_cells, 32768, stream_of(stream));
+    if(used_fallback)
+      *used_fallback = bad > 0 ? 1 : 0;
   });
 }
 

@@ -48,6 +48,19 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
     }
   }
   tables_.upload(tab);
+  {
+    // FP64 tables of the divergence right-hand side: S, Sd = Dq S, then weights and points
+    std::vector<double> dt(shape_.S);
+    std::vector<double> Sd((size_t)n * n, 0.0);
+    for(int q = 0; q < n; ++q)
+      for(int i = 0; i < n; ++i)
+        for(int p = 0; p < n; ++p)
+          Sd[q * n + i] += shape_.Dq[q * n + p] * shape_.S[p * n + i];
+    dt.insert(dt.end(), Sd.begin(), Sd.end());
+    dt.insert(dt.end(), shape_.qw.begin(), shape_.qw.end());
+    dt.insert(dt.end(), shape_.qp.begin(), shape_.qp.end());
+    div_tables_.upload(dt);
+  }
   auto convert = [](std::vector<double> const & v) {
     return std::vector<Number>(v.begin(), v.end());
   };
@@ -127,6 +140,64 @@ void LaplaceOperator<Number>::rhs_sine(double omega, double * b, hipStream_t s) 
   std::vector<double> hb;
   assemble_rhs_sine(*grid_, shape_, geo_, omega, hb);
   HIP_CHECK(hipMemcpyAsync(b, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template<typename Number>
+bool LaplaceOperator<Number>::is_singular() const
+{
+  BoxTopology const & t = grid_->topology();
+  for(int d = 0; d < 3; ++d)
+    for(int s = 0; s < 2; ++s)
+      if(!t.periodic[d] && t.bc[2 * d + s] == EXADG_BC_DIRICHLET)
+        return false;
+  return true;
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::rhs_divergence_fourier(double const * modes_host, int n_modes,
+                                                     double t, double * b, hipStream_t s) const
+{
+  if(b == nullptr || modes_host == nullptr || n_modes < 1)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "invalid Fourier field");
+  if(!grid_->affine())
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "the divergence right-hand side needs an affine mesh");
+  // per-call mode data: n, d, and c = A exp(i (phi + omega t)), so that the kernel evaluates
+  // u = sum A d cos(2 pi n.x + phi + omega t) = Re sum d c exp(i 2 pi n.x)
+  std::vector<double> mh((size_t)8 * n_modes);
+  int                 K = 0;
+  for(int m = 0; m < n_modes; ++m)
+  {
+    double const * q = modes_host + 10 * m;
+    for(int d = 0; d < 3; ++d)
+    {
+      if(q[d] != std::round(q[d]) || std::abs(q[d]) > 16)
+        EXADG_THROW(EXADG_ERR_ARGUMENT, "wave vectors must be integer with |n_d| <= 16");
+      K = std::max(K, (int)std::abs(q[d]));
+      mh[8 * m + d]     = q[d];
+      mh[8 * m + 3 + d] = q[3 + d];
+    }
+    double const ph = q[7] + q[8] * t;
+    mh[8 * m + 6]   = q[6] * std::cos(ph);
+    mh[8 * m + 7]   = q[6] * std::sin(ph);
+  }
+  if(modes_.size() != mh.size())
+    modes_.resize(mh.size());
+  HIP_CHECK(hipMemcpyAsync(modes_.data(), mh.data(), mh.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  int const    n    = shape_.n;
+  double const h[3] = {grid_->h(0), grid_->h(1), grid_->h(2)};
+  hipError_t   e    = launch_divergence_rhs(b, modes_.data(), n_modes, K, grid_->topology(),
+                                            grid_->mesh().lower, h, n, div_tables_.data(),
+                                            div_tables_.data() + n * n,
+                                            div_tables_.data() + 2 * n * n, grid_->n_cells(), s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  if(is_singular())
+  {
+    if(!reducer_)
+      reducer_ = std::make_unique<Reducer>();
+    vec::subtract_mean<double>(*reducer_, b, m(), s);
+  }
   HIP_CHECK(hipStreamSynchronize(s));
 }
 

@@ -7,8 +7,10 @@
 
 #include "common.h"
 #include "grid.h"
+#include "guess_kernels.h"
 #include "laplace_kernels.h"
 #include "shape_info.h"
+#include "vector_ops.h"
 
 namespace ExaDG
 {
@@ -35,6 +37,11 @@ public:
   virtual void    calculate_diagonal(void * diag, hipStream_t s) const         = 0;
   virtual void    calculate_inverse_diagonal(void * diag, hipStream_t s) const = 0;
   virtual void    rhs_sine(double omega, double * b, hipStream_t s) const      = 0;
+  // weak DG divergence (central flux, no-slip walls) of a Fourier velocity field; mean
+  // removed when the operator is singular (affine meshes)
+  virtual void rhs_divergence_fourier(double const * modes_host, int n_modes, double t, double * b,
+                                      hipStream_t s) const = 0;
+  virtual bool is_singular() const                                             = 0;
   virtual std::shared_ptr<StructuredGrid const> grid_ptr() const               = 0;
   virtual double                                ip_factor() const              = 0;
 };
@@ -78,6 +85,9 @@ public:
   }
   // Poisson::Operator::rhs for the manufactured sine solution (FP64 output)
   void rhs_sine(double omega, double * b, hipStream_t s) const override;
+  void rhs_divergence_fourier(double const * modes_host, int n_modes, double t, double * b,
+                              hipStream_t s) const override;
+  bool is_singular() const override;
 
   StructuredGrid const & grid() const { return *grid_; }
   std::shared_ptr<StructuredGrid const> grid_ptr() const override { return grid_; }
@@ -94,6 +104,9 @@ private:
   GeometryData                          geo_;  // host copy (tau, and metric for curved)
   int                                   n3_;
   DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
+  DeviceBuffer<double>                  div_tables_;          // S, Sd, weights|points (FP64)
+  mutable DeviceBuffer<double>          modes_;
+  mutable std::unique_ptr<Reducer>      reducer_;
 };
 
 std::unique_ptr<LaplaceOperatorBase> create_laplace_operator(exadg_mesh_desc const & mesh,

@@ -1,0 +1,113 @@
+"""A sequence of pressure-Poisson solves with a learned initial guess.
+
+The sequence mimics the pressure step of the dual-splitting scheme
+(time_int_bdf_dual_splitting.cpp:489-540): at every time step t_s = s dt the right-hand side is
+the weak DG divergence of a solenoidal Fourier velocity field with no-slip walls, the initial
+guess is extrapolated from the previous solutions (orders ramp up 1, 2, 3 at start-up as in
+ExtrapolationConstants), and CG + hybrid p-multigrid runs to ||r|| <= rel_tol ||b||. The
+operator is singular on a periodic-x/y, walled-z box (Neumann pressure on the walls), so the
+right-hand side and every solution have their mean removed.
+
+Arms on the held-out steps, all from the same exact history of previous solutions:
+  zero guess, order-3 extrapolation (the reference's guess), learned guess (GuessNet).
+The learned guess is trained on the first steps only (no leakage).
+"""
+import time
+
+import numpy as np
+
+
+def setup_box(ex, n_cells, degree, precision_mg):
+    mesh = ex.mesh_desc(n_cells, periodic=(1, 1, 0))
+    bc = ex.bc_desc((ex.NEUMANN,) * 6)
+    op = ex.LaplaceOperator(mesh, bc, degree)
+    prec = ex.Preconditioner(op, ex.PRECOND_MULTIGRID,
+                             ex.mg_desc(degree, precision=precision_mg, singular=True))
+    return op, prec
+
+
+def generate(ex, torch, op, prec, modes, n_steps, dt=0.01, rel_tol=1e-10, order=3, log=None):
+    """Solve the whole sequence with the extrapolated guess. Returns FP64 solutions P and
+    right-hand sides Bv (n_steps x n, device) and per-step iterations / times."""
+    n = op.n
+    P = torch.empty((n_steps, n), dtype=torch.float64, device="cuda")
+    Bv = torch.empty((n_steps, n), dtype=torch.float64, device="cuda")
+    its, times = [], []
+    for s in range(n_steps):
+        op.rhs_divergence_fourier(modes, s * dt, Bv[s])
+        x = P[s]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        o = min(s, order)
+        if o == 0:
+            x.zero_()
+        else:
+            ex.extrapolate([P[s - 1 - i] for i in range(o)], x)
+        info = ex.cg_solve(op, Bv[s], x, prec, rel_tol=rel_tol, rel_mode=1)
+        ex.subtract_mean(x)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        its.append(info["iterations"])
+        if log and (s % 50 == 0 or s == n_steps - 1):
+            log(f"step {s}: {info['iterations']} iterations")
+    return P, Bv, its, times
+
+
+def solve_from(ex, torch, op, prec, b, x0, rel_tol=1e-10):
+    x = x0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = ex.cg_solve(op, b, x, prec, rel_tol=rel_tol, rel_mode=1)
+    torch.cuda.synchronize()
+    return info, time.perf_counter() - t0, x
+
+
+def ab_test(ex, torch, op, prec, net, P, Bv, s_begin, s_end, rel_tol=1e-10,
+            arms=("zero", "extrapolation", "learned")):
+    """Iterations and times of the guesses `arms` on steps [s_begin, s_end). Every arm starts from
+    the same exact history; the time of an arm includes computing its guess."""
+    H = 3 if net is None else net.history
+    res = {a: [] for a in arms}
+    fallbacks = 0
+    x = torch.empty_like(P[0])
+    for s in range(s_begin, s_end):
+        hist = [P[s - 1 - i] for i in range(H)]
+        b = Bv[s]
+        bn = float(torch.linalg.norm(b))
+        for arm in arms:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if arm == "zero":
+                x.zero_()
+            elif arm == "extrapolation":
+                ex.extrapolate(hist, x)
+            else:
+                fallbacks += int(net.predict(hist, b, x))
+            info = ex.cg_solve(op, b, x, prec, rel_tol=rel_tol, rel_mode=1)
+            torch.cuda.synchronize()
+            res[arm].append((info["iterations"], time.perf_counter() - t0, info["r0"] / bn,
+                             info["r_final"] / bn))
+    out = {}
+    for k, v in res.items():
+        out[k] = {"mean_iterations": float(np.mean([r[0] for r in v])),
+                  "total_time_s": float(sum(r[1] for r in v)),
+                  "mean_initial_rel_residual": float(np.mean([r[2] for r in v])),
+                  "max_final_rel_residual": float(max(r[3] for r in v))}
+    if "learned" in out:
+        out["learned"]["fallbacks"] = fallbacks
+    return out
+
+
+def train(ex, net, torch, op, P, Bv, t_end, epochs, lr, loss=None, batch_cells=4096, seed=7):
+    """Train on steps [0, t_end) of the FP64 sequence; one call per epoch so that the per-epoch
+    losses are returned."""
+    if loss is None:
+        loss = ex.GUESS_LOSS_ENERGY
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = []
+    for ep in range(epochs):
+        losses.append(net.train(P, Bv, 0, t_end, 1, op=op, loss=loss, batch_cells=batch_cells, lr=lr,
+                                seed=seed + ep))
+    torch.cuda.synchronize()
+    return losses, time.perf_counter() - t0

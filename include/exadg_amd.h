@@ -195,6 +195,80 @@ typedef struct exadg_solve_info
 int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
                    const exadg_solver_data * data, exadg_solve_info * info, void * stream);
 
+/* ---------------------------------------------------------------- pressure sequences */
+
+/* Seeded solenoidal Fourier velocity field u(x,t) = sum_m A_m d_m cos(2 pi n_m.x + phi_m + omega_m t)
+ * over the integer shell k_min <= |n_m| <= k_max (one of each pair +-n): d_m = (n_m/|n_m|) x xi_m
+ * with xi_m uniform on the unit sphere, A_m = |n_m|^{-5/3}, phi_m uniform in [0, 2 pi),
+ * omega_m = |n_m|. modes (host): 10 doubles per mode, n(3) d(3) A phi omega 0; with
+ * modes == NULL only *n_modes (the shell size) is returned. */
+int exadg_fourier_modes(uint64_t seed, int k_min, int k_max, int max_modes, double * modes,
+                        int * n_modes);
+
+/* Right-hand side of the pressure step: the weak DG divergence with central flux and
+ * no-slip walls (divergence_operator.h:63-68, divergence_operator.cpp:168-298) of the Fourier
+ * field at time t, mean removed when the operator is singular (no Dirichlet side). Affine
+ * meshes; b is FP64 (device). */
+int exadg_laplace_rhs_divergence_fourier(exadg_laplace * op, const double * modes, int n_modes,
+                                         double t, double * b, void * stream);
+
+/* v -= mean(v) (FP64, device): the pressure level of a singular problem. */
+int exadg_vector_subtract_mean(double * v, int64_t n, void * stream);
+
+/* Initial guess by extrapolation, x0 = sum_i beta_i p^{n-i} with the constants of
+ * ExtrapolationConstants::set_constant_time_step (extrapolation_constants.cpp:55-93), as used for
+ * the pressure at time_int_bdf_dual_splitting.cpp:489-501. history: host array of `order`
+ * device pointers, newest first. */
+int exadg_extrapolate(int order, const double * const * history, double * x0, int64_t n,
+                      void * stream);
+
+/* Learned initial guess (no counterpart in the reference; it replaces the extrapolated guess
+ * above, which remains its fallback). A per-cell MLP, FP32, of `width` neurons in two hidden
+ * ReLU layers. Input on the cell's (k+1)^3 DoFs: the backward differences of orders
+ * 0..history-1 of p^{n-1}, p^{n-2}, ... and b^n, each divided by its scale. Output: the
+ * correction of the order-`history` extrapolation divided by s_t. */
+typedef struct exadg_guess exadg_guess;
+int exadg_guess_create(int degree, int history, int width, uint64_t seed, exadg_guess ** out);
+int exadg_guess_destroy(exadg_guess * g);
+int exadg_guess_n_params(const exadg_guess * g, int * n_params, int * n_features, int * n_outputs);
+/* copy the flat FP32 parameters [W1 | b1 | W2 | b2 | W3 | b3] (W row-major, out x in) to / from
+ * n_params floats of device memory; stream-ordered, returns after the copy */
+int exadg_guess_get_params(const exadg_guess * g, float * dst, void * stream);
+int exadg_guess_set_params(exadg_guess * g, const float * src, void * stream);
+/* scales [s_0 .. s_{history-1}, s_b, s_t] (n = history + 2): the RMS of the j-th backward
+ * difference of the history, of the rhs and of the extrapolation error; train() sets them */
+int exadg_guess_set_scales(exadg_guess * g, const double * scales, int n);
+int exadg_guess_get_scales(const exadg_guess * g, double * scales, int n);
+/* mean squared error over B samples of features X (B x n_features) and targets T (B x n_outputs),
+ * device FP32; gradient (n_params floats) copied to grad if not NULL. */
+int exadg_guess_loss_and_grad(exadg_guess * g, const float * X, const float * T, int B, float * grad,
+                              double * loss, void * stream);
+/* Training losses of the learned guess, each normalized per step by the same quantity of the
+ * extrapolation error (1 = as good as the extrapolation): L2 = sum of squared DoF errors
+ * (per-cell minibatches), ENERGY = e^T A e and RESIDUAL = |A e|^2 (whole field per step, A the
+ * FP64 operator of the sequence). */
+enum exadg_guess_loss
+{
+  EXADG_GUESS_LOSS_L2       = 0,
+  EXADG_GUESS_LOSS_ENERGY   = 1,
+  EXADG_GUESS_LOSS_RESIDUAL = 2
+};
+
+/* Adam training on a stored FP64 sequence P[t][n_dofs] (solutions), Bv[t][n_dofs] (right-hand
+ * sides), device, step-major, over steps [t_begin + history, t_end). op: the FP64 operator
+ * (NULL allowed for L2). The first call on a network sets the scales from the same steps.
+ * loss = mean normalized loss over the last epoch. */
+int exadg_guess_train(exadg_guess * g, exadg_laplace * op, int loss_kind, const double * P,
+                      const double * Bv, int64_t n_dofs, int t_begin, int t_end, int epochs,
+                      int batch_cells, double lr, uint64_t seed, double * loss, void * stream);
+int exadg_guess_evaluate(exadg_guess * g, exadg_laplace * op, int loss_kind, const double * P,
+                         const double * Bv, int64_t n_dofs, int t_begin, int t_end, int batch_cells,
+                         double * loss, void * stream);
+/* x0 = extrapolation + network correction; *used_fallback = 1 if a non-finite network output
+ * forced the plain extrapolation. history: host array of `history` device pointers (FP64). */
+int exadg_guess_predict(exadg_guess * g, const double * const * history, const double * rhs,
+                        double * x0, int64_t n_cells, int * used_fallback, void * stream);
+
 #ifdef __cplusplus
 }
 #endif

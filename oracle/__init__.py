@@ -67,6 +67,8 @@ def lib():
         L.oracle_laplace_n_dofs.argtypes = [ctypes.c_void_p]
         L.oracle_laplace_vmult.argtypes = [ctypes.c_void_p, dp, dp]
         L.oracle_laplace_rhs_sine.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        L.oracle_laplace_rhs_divergence_fourier.argtypes = [ctypes.c_void_p, dp, ctypes.c_int,
+                                                            ctypes.c_double, dp]
         L.oracle_laplace_error_sine.restype = ctypes.c_double
         L.oracle_laplace_error_sine.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
         L.oracle_laplace_diagonal.argtypes = [ctypes.c_void_p, dp]
@@ -150,6 +152,14 @@ class LaplaceOperator:
     def rhs_sine(self, omega):
         b = np.empty(self.n)
         lib().oracle_laplace_rhs_sine(self._h, omega, _p(b))
+        return b
+
+    def rhs_divergence_fourier(self, modes, t):
+        """Weak DG divergence (central flux, zero wall flux) of the Fourier field `modes`
+        (rows n(3) d(3) A phi omega pad) at time t."""
+        m = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1)
+        b = np.empty(self.n)
+        lib().oracle_laplace_rhs_divergence_fourier(self._h, _p(m), m.size // 10, t, _p(b))
         return b
 
     def error_sine(self, omega, u):

@@ -58,6 +58,12 @@ void oracle_laplace_rhs_sine(void * h, double omega, double * b)
   static_cast<LaplaceOperator *>(h)->rhs_sine(omega, b);
 }
 
+void oracle_laplace_rhs_divergence_fourier(void * h, double const * modes, int n_modes, double t,
+                                           double * b)
+{
+  static_cast<LaplaceOperator *>(h)->rhs_divergence_fourier(modes, n_modes, t, b);
+}
+
 double oracle_laplace_error_sine(void * h, double omega, double const * u)
 {
   return static_cast<LaplaceOperator *>(h)->error_sine(omega, u);

@@ -779,6 +779,103 @@ void LaplaceOperator::rhs_sine(double omega, double * b) const
   }
 }
 
+namespace
+{
+void fourier_velocity(double const * modes, int n_modes, double t, double const * x, double * u)
+{
+  u[0] = u[1] = u[2] = 0.0;
+  for(int m = 0; m < n_modes; ++m)
+  {
+    double const * q  = modes + 10 * m;
+    double const   ph = 2.0 * M_PI * (q[0] * x[0] + q[1] * x[1] + q[2] * x[2]) + q[7] + q[8] * t;
+    double const   a  = q[6] * std::cos(ph);
+    for(int d = 0; d < 3; ++d)
+      u[d] += a * q[3 + d];
+  }
+}
+}  // namespace
+
+void LaplaceOperator::rhs_divergence_fourier(double const * modes, int n_modes, double time,
+                                             double * b) const
+{
+  int64_t const nc = mesh.n_cells;
+  int const     n2 = n * n;
+  // cell term -(grad phi_i, u): reference-gradient data J^{-1} u JxW, tested with Sd
+#pragma omp parallel
+  {
+    std::vector<double> g(3 * n3), out(n3);
+    double const *      ops[3][3];
+    for(int j = 0; j < 3; ++j)
+      for(int k = 0; k < 3; ++k)
+        ops[j][k] = (j == k) ? t.Sd.data() : t.S.data();
+#pragma omp for schedule(static)
+    for(int64_t c = 0; c < nc; ++c)
+    {
+      for(int qz = 0; qz < n; ++qz)
+        for(int qy = 0; qy < n; ++qy)
+          for(int qx = 0; qx < n; ++qx)
+          {
+            int const    q     = qx + n * (qy + n * qz);
+            double const xi[3] = {t.gp[qx], t.gp[qy], t.gp[qz]};
+            double       x[3], J[9], u[3];
+            mesh.map_point(c, xi, x, J);
+            fourier_velocity(modes, n_modes, time, x, u);
+            double const * Ji  = &cell_jinv[((size_t)c * n3 + q) * 9];
+            double const   jxw = cell_jxw[(size_t)c * n3 + q];
+            for(int j = 0; j < 3; ++j)
+              g[j * n3 + q] = -jxw * (Ji[j * 3 + 0] * u[0] + Ji[j * 3 + 1] * u[1] + Ji[j * 3 + 2] * u[2]);
+          }
+      std::fill(out.begin(), out.end(), 0.0);
+      for(int j = 0; j < 3; ++j)
+        tensor_apply_T(n, n, n, n, ops[j][0], ops[j][1], ops[j][2], g.data() + j * n3, out.data());
+      for(int i = 0; i < n3; ++i)
+        b[c * n3 + i] = out[i];
+    }
+  }
+  // face term, one visit per interior/periodic face (upper side of cell c): the flux u.n at
+  // the face points goes to c with +, to the neighbour with - (its outward normal is -n).
+  // Boundary faces are walls: {u} = 0, no contribution.
+  std::vector<double> jm(9 * n2), wm(n2), nm(3 * n2), xm(3 * n2), fv(n2), fneg(n2),
+    zero(3 * n2, 0.0), out(n3);
+  for(int64_t c = 0; c < nc; ++c)
+    for(int dir = 0; dir < 3; ++dir)
+    {
+      int64_t const nb = mesh.neighbor(c, dir, 1);
+      if(nb < 0)
+        continue;
+      face_geometry(c, dir, 1, jm.data(), wm.data(), nm.data(), xm.data());
+      for(int q = 0; q < n2; ++q)
+      {
+        double u[3];
+        fourier_velocity(modes, n_modes, time, &xm[3 * q], u);
+        fv[q]   = (u[0] * nm[3 * q] + u[1] * nm[3 * q + 1] + u[2] * nm[3 * q + 2]) * wm[q];
+        fneg[q] = -fv[q];
+      }
+      std::fill(out.begin(), out.end(), 0.0);
+      face_integrate(dir, 1, fv.data(), zero.data(), out.data());
+      for(int i = 0; i < n3; ++i)
+        b[c * n3 + i] += out[i];
+      std::fill(out.begin(), out.end(), 0.0);
+      face_integrate(dir, 0, fneg.data(), zero.data(), out.data());
+      for(int i = 0; i < n3; ++i)
+        b[nb * n3 + i] += out[i];
+    }
+  bool singular = true;
+  for(int d = 0; d < 3; ++d)
+    if(!mesh.d.periodic[d] && (mesh.d.boundary_type[2 * d] == 0 || mesh.d.boundary_type[2 * d + 1] == 0))
+      singular = false;
+  if(singular)
+  {
+    int64_t const nd   = n_dofs();
+    double        mean = 0;
+    for(int64_t i = 0; i < nd; ++i)
+      mean += b[i];
+    mean /= (double)nd;
+    for(int64_t i = 0; i < nd; ++i)
+      b[i] -= mean;
+  }
+}
+
 double LaplaceOperator::error_sine(double omega, double const * u) const
 {
   // calculate_error with QGauss(k+3) (error_calculation.cpp:38-103), relative L2 norm

@@ -122,6 +122,11 @@ struct LaplaceOperator
 
   // b for u* = prod sin(omega x_d): f = 3 omega^2 u*, Dirichlet g = u*, Neumann h = grad u*.n
   void rhs_sine(double omega, double * b) const;
+  // weak DG divergence of u = sum_m A_m d_m cos(2 pi n_m.x + phi_m + omega_m t) (modes: 10
+  // doubles each, n d A phi omega pad) with central flux on interior/periodic faces and zero flux
+  // on walls (no slip): b_i = -(grad phi_i, u)_K + sum_F ({u}.n_K, phi_i)_F; mean removed when
+  // no side is Dirichlet (divergence_operator.cpp:168-298, rhs of the pressure Poisson step)
+  void rhs_divergence_fourier(double const * modes, int n_modes, double t, double * b) const;
   // relative L2 error vs prod sin(omega x_d) with QGauss(k+3)
   double error_sine(double omega, double const * u) const;
   // exact diagonal by unit-vector probing of the local cell/face matrices

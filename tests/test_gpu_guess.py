@@ -1,0 +1,160 @@
+"""GPU checks of the pressure-sequence kernels and the learned initial guess (C ABI).
+
+* The divergence right-hand side kernel equals the CPU oracle to 1e-12 (relative, FP64) on
+  singular (periodic x/y, Neumann walls) and non-singular boxes.
+* Extrapolation equals sum beta_i p^{n-i} (extrapolation_constants.cpp:62-83) to 1e-15.
+* The network's forward/backward kernels give the numpy oracle's loss and gradient
+  (FP32 vs FP64: 2e-4 relative in the max norm), on shapes that are not tile multiples.
+* Features and predict() match the oracle's features and forward pass.
+* A non-finite network output falls back to the extrapolated guess.
+* Training lowers the loss, and on a held-out part of a pressure sequence the learned guess
+  starts CG from a smaller residual than the extrapolation, at no more iterations; every arm
+  reaches 1e-10 vs ||b||.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import guess as og
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("k,nc,periodic,bc", [
+    (3, 4, (1, 1, 0), (1,) * 6),
+    (4, 3, (1, 1, 0), (1,) * 6),
+    (2, (3, 4, 2), (0, 1, 0), (0, 1, 1, 1, 0, 1)),
+    (5, 2, (1, 1, 1), (0,) * 6),
+])
+def test_divergence_rhs_vs_oracle(torch_cuda, product, k, nc, periodic, bc):
+    torch = torch_cuda
+    modes = product.fourier_modes(21, 1, 3)
+    mesh = product.mesh_desc(nc, periodic=periodic)
+    op = product.LaplaceOperator(mesh, product.bc_desc(bc), k)
+    b = torch.empty(op.n, dtype=torch.float64, device="cuda")
+    op.rhs_divergence_fourier(modes, 0.37, b)
+    torch.cuda.synchronize()
+    ncs = nc if isinstance(nc, tuple) else (nc,) * 3
+    ref = oracle.LaplaceOperator(oracle.make_mesh(ncs, periodic=periodic, boundary_type=bc), k)
+    bo = ref.rhs_divergence_fourier(modes, 0.37)
+    assert _rel(b.cpu().numpy(), bo) < 1e-12
+
+
+def test_divergence_rhs_rejects_bad_modes(torch_cuda, product):
+    torch = torch_cuda
+    op = product.LaplaceOperator(product.mesh_desc(2), product.bc_desc(), 2)
+    b = torch.empty(op.n, dtype=torch.float64, device="cuda")
+    bad = product.fourier_modes(1, 1, 2)
+    bad[0, 0] = 0.5  # non-integer wave vector
+    with pytest.raises(product.ExaDGError):
+        op.rhs_divergence_fourier(bad, 0.0, b)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_extrapolate(torch_cuda, product, order):
+    torch = torch_cuda
+    g = torch.Generator().manual_seed(order)
+    hist = [torch.rand(1000, dtype=torch.float64, generator=g).cuda() for _ in range(order)]
+    x0 = torch.empty_like(hist[0])
+    product.extrapolate(hist, x0)
+    torch.cuda.synchronize()
+    ref = og.extrapolate([h.cpu().numpy() for h in hist])
+    assert np.max(np.abs(x0.cpu().numpy() - ref)) < 1e-15 * 8
+
+
+@pytest.mark.parametrize("degree,history,width,B", [(1, 3, 40, 70), (2, 3, 72, 133), (3, 2, 128, 300)])
+def test_loss_and_grad_vs_oracle(torch_cuda, product, degree, history, width, B):
+    torch = torch_cuda
+    net = product.GuessNet(degree, history=history, width=width, seed=11)
+    n3 = (degree + 1) ** 3
+    assert net.n_features == (history + 1) * n3 and net.n_outputs == n3
+    rng = np.random.default_rng(degree)
+    p = rng.uniform(-0.3, 0.3, net.n_params).astype(np.float32)
+    net.set_params(torch.from_numpy(p))
+    X = rng.normal(size=(B, net.n_features)).astype(np.float32)
+    T = rng.normal(size=(B, n3)).astype(np.float32)
+    loss, g = net.loss_and_grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
+    lo, go = og.loss_and_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64),
+                              width, n3)
+    assert loss == pytest.approx(lo, rel=1e-4)
+    g = g.cpu().numpy()
+    assert np.max(np.abs(g - go)) < 2e-4 * np.max(np.abs(go))
+    # every block of the flat layout is touched (no block silently zero)
+    W1, b1, W2, b2, W3, b3 = og.unpack(g, net.n_features, width, n3)
+    for blk in (W1, b1, W2, b2, W3, b3):
+        assert np.max(np.abs(blk)) > 0
+
+
+def _sequence(torch, product, n_cells, degree, n_steps, k_max=3, dt=0.01, seed=5):
+    from exadg_amd import pressure_sequence as ps
+    op, prec = ps.setup_box(product, n_cells, degree, product.FP32)
+    modes = product.fourier_modes(seed, 1, k_max)
+    P, Bv, its, _ = ps.generate(product, torch, op, prec, modes, n_steps, dt=dt)
+    return op, prec, P, Bv, its
+
+
+def test_predict_matches_oracle_and_falls_back(torch_cuda, product):
+    torch = torch_cuda
+    op, prec, P, Bv, its = _sequence(torch, product, 3, 2, 8)
+    net = product.GuessNet(2, history=3, width=32, seed=3)
+    # the first training call fits the scales; use them for the oracle features
+    net.train(P, Bv, 0, 8, 1, batch_cells=9, lr=1e-3)
+    sc = net.scales()
+    assert len(sc) == 5 and all(v > 0 for v in sc)
+    s = 7
+    hist = [P[s - 1], P[s - 2], P[s - 3]]
+    x0 = torch.empty_like(P[0])
+    assert not net.predict(hist, Bv[s], x0)
+    torch.cuda.synchronize()
+    n3 = 27
+    X, _ = og.features([h.cpu().numpy() for h in hist], Bv[s].cpu().numpy(), n3, sc)
+    Y, _ = og.forward(net.params().cpu().numpy().astype(np.float64), X, 32, n3)
+    ref = og.extrapolate([h.cpu().numpy() for h in hist]) + sc[-1] * Y.reshape(-1)
+    ex = og.extrapolate([h.cpu().numpy() for h in hist])
+    # FP32 network: the correction matches to FP32 accuracy, the total to far below it
+    corr = x0.cpu().numpy() - ex
+    assert _rel(corr, ref - ex) < 1e-4
+    # non-finite output -> plain extrapolation
+    bad = net.params()
+    bad[-1] = float("nan")
+    net.set_params(bad)
+    assert net.predict(hist, Bv[s], x0)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(x0.cpu().numpy() - ex)) <= 1e-15 * np.max(np.abs(ex)) * 8
+
+
+def test_learned_guess_ab(torch_cuda, product):
+    """Train on the first 75 steps of a 100-step sequence (8^3 cells, k=3), compare the guesses on
+    the last 25. The network is trained on the residual loss |A e|^2, the quantity CG's stopping
+    test measures; 1 = the order-3 extrapolation's residual."""
+    from exadg_amd import pressure_sequence as ps
+    torch = torch_cuda
+    op, prec, P, Bv, its = _sequence(torch, product, 8, 3, 100, k_max=3)
+    assert max(its) < 60
+    net = product.GuessNet(3, history=3, width=128, seed=1)
+    losses, _ = ps.train(product, net, torch, op, P, Bv, 75, epochs=30, lr=1e-3,
+                         loss=product.GUESS_LOSS_RESIDUAL)
+    assert losses[-1] < 0.5 * losses[0]
+    held_out = net.evaluate(P, Bv, 72, 100, op=op, loss=product.GUESS_LOSS_RESIDUAL)
+    r = ps.ab_test(product, torch, op, prec, net, P, Bv, 75, 100)
+    print("losses", losses[0], losses[-1], "held-out", held_out, r)
+    # held-out loss below 1: the learned guess leaves a smaller residual than the extrapolation
+    assert held_out < 0.8
+    assert r["learned"]["mean_initial_rel_residual"] < r["extrapolation"]["mean_initial_rel_residual"]
+    assert r["learned"]["mean_iterations"] <= r["extrapolation"]["mean_iterations"] + 0.25
+    assert r["extrapolation"]["mean_iterations"] < r["zero"]["mean_iterations"]
+    for arm in ("zero", "extrapolation", "learned"):
+        assert r[arm]["max_final_rel_residual"] <= 1e-10
+    assert r["learned"]["fallbacks"] == 0
+
+
+def test_operator_loss_needs_operator(torch_cuda, product):
+    torch = torch_cuda
+    net = product.GuessNet(2, history=3, width=16, seed=1)
+    P = torch.zeros((6, 27 * 8), dtype=torch.float64, device="cuda")
+    with pytest.raises(product.ExaDGError):
+        net.train(P, P.clone(), 0, 6, 1, loss=product.GUESS_LOSS_ENERGY)
