@@ -55,6 +55,7 @@ class MGDesc(ctypes.Structure):
         ("coarse_rel_tol", ctypes.c_double),
         ("coarse_max_iter", ctypes.c_int32),
         ("operator_is_singular", ctypes.c_int32),
+        ("capture_graph", ctypes.c_int32),
     ]
 
 
@@ -75,7 +76,7 @@ COARSE_CG, COARSE_CHEBYSHEV = 0, 1
 
 def mg_desc(degree, degrees=None, precision=FP32, coarse_q1=True, coarse_solver=COARSE_CG,
             coarse_rel_tol=1e-3, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=20,
-            singular=False):
+            singular=False, capture_graph=False):
     """MultigridData; default p-sequence Bisect (k, k/2, ..., 1) + continuous Q1 level."""
     d = MGDesc()
     check(lib().exadg_mg_desc_default(degree, ctypes.byref(d)))
@@ -91,6 +92,7 @@ def mg_desc(degree, degrees=None, precision=FP32, coarse_q1=True, coarse_solver=
     d.smoothing_range = smoothing_range
     d.eig_cg_iterations = eig_cg_iterations
     d.operator_is_singular = int(singular)
+    d.capture_graph = int(capture_graph)
     return d
 
 

@@ -211,6 +211,7 @@ int exadg_mg_desc_default(int degree, exadg_mg_desc * out)
     d.coarse_rel_tol       = 1e-3;
     d.coarse_max_iter      = 10000;
     d.operator_is_singular = 0;
+    d.capture_graph        = 0;
     *out                   = d;
   });
 }
@@ -248,6 +249,7 @@ int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_des
       d.coarse_rel_tol       = mg->coarse_rel_tol;
       d.coarse_max_iter      = mg->coarse_max_iter;
       d.operator_is_singular = mg->operator_is_singular != 0;
+      d.capture_graph        = mg->capture_graph != 0;
       require(d.smoother_degree >= 1 && d.smoothing_range > 1.0 && d.eig_cg_iterations >= 1,
               "invalid smoother data");
       require(d.coarse_rel_tol > 0.0 && d.coarse_rel_tol < 1.0 && d.coarse_max_iter >= 1,

@@ -76,6 +76,8 @@ GuessNet::GuessNet(int degree, int history, int width, uint64_t seed)
   m_.upload(z);
   v_.upload(z);
   partial_.resize(1024 * 6);
+  // split-K partials of the weight gradients: up to 64 slices of the largest weight matrix
+  split_.resize((size_t)64 * std::max<int64_t>((int64_t)width * F_, (int64_t)n3_ * width));
   loss_.resize(1);
   bad_.resize(1);
 }
@@ -99,9 +101,9 @@ float const * GuessNet::forward(float const * X, int B, hipStream_t s)
 {
   ensure_workspace(B);
   int const w = width_;
-  check_launch(launch_sgemm(B, w, F_, X, F_, 1, W1(), 1, F_, H1_.data(), w, 1.f, false, b1(), true, s));
-  check_launch(launch_sgemm(B, w, w, H1_.data(), w, 1, W2(), 1, w, H2_.data(), w, 1.f, false, b2(), true, s));
-  check_launch(launch_sgemm(B, n3_, w, H2_.data(), w, 1, W3(), 1, w, Y_.data(), n3_, 1.f, false, b3(), false, s));
+  check_launch(launch_sgemm(B, w, F_, X, F_, 1, W1(), 1, F_, H1_.data(), w, 1.f, false, b1(), true, nullptr, 0, s));
+  check_launch(launch_sgemm(B, w, w, H1_.data(), w, 1, W2(), 1, w, H2_.data(), w, 1.f, false, b2(), true, nullptr, 0, s));
+  check_launch(launch_sgemm(B, n3_, w, H2_.data(), w, 1, W3(), 1, w, Y_.data(), n3_, 1.f, false, b3(), false, nullptr, 0, s));
   return Y_.data();
 }
 
@@ -109,17 +111,22 @@ void GuessNet::backward(float const * X, int B, hipStream_t s)
 {
   int const w = width_;
   // output layer
-  check_launch(launch_sgemm(n3_, w, B, dY_.data(), 1, n3_, H2_.data(), w, 1, gW(W3()), w, 1.f, false, nullptr, false, s));
+  check_launch(launch_sgemm(n3_, w, B, dY_.data(), 1, n3_, H2_.data(), w, 1, gW(W3()), w, 1.f, false, nullptr, false, split_.data(),
+                            (int64_t)split_.size(), s));
   check_launch(launch_column_sum(dY_.data(), B, n3_, gW(b3()), s));
-  check_launch(launch_sgemm(B, w, n3_, dY_.data(), n3_, 1, W3(), w, 1, dH2_.data(), w, 1.f, false, nullptr, false, s));
+  check_launch(launch_sgemm(B, w, n3_, dY_.data(), n3_, 1, W3(), w, 1, dH2_.data(), w, 1.f, false, nullptr, false, split_.data(),
+                            (int64_t)split_.size(), s));
   check_launch(launch_relu_backward(dH2_.data(), H2_.data(), (int64_t)B * w, s));
   // hidden layer 2
-  check_launch(launch_sgemm(w, w, B, dH2_.data(), 1, w, H1_.data(), w, 1, gW(W2()), w, 1.f, false, nullptr, false, s));
+  check_launch(launch_sgemm(w, w, B, dH2_.data(), 1, w, H1_.data(), w, 1, gW(W2()), w, 1.f, false, nullptr, false, split_.data(),
+                            (int64_t)split_.size(), s));
   check_launch(launch_column_sum(dH2_.data(), B, w, gW(b2()), s));
-  check_launch(launch_sgemm(B, w, w, dH2_.data(), w, 1, W2(), w, 1, dH1_.data(), w, 1.f, false, nullptr, false, s));
+  check_launch(launch_sgemm(B, w, w, dH2_.data(), w, 1, W2(), w, 1, dH1_.data(), w, 1.f, false, nullptr, false, split_.data(),
+                            (int64_t)split_.size(), s));
   check_launch(launch_relu_backward(dH1_.data(), H1_.data(), (int64_t)B * w, s));
   // hidden layer 1
-  check_launch(launch_sgemm(w, F_, B, dH1_.data(), 1, w, X, F_, 1, gW(W1()), F_, 1.f, false, nullptr, false, s));
+  check_launch(launch_sgemm(w, F_, B, dH1_.data(), 1, w, X, F_, 1, gW(W1()), F_, 1.f, false, nullptr, false, split_.data(),
+                            (int64_t)split_.size(), s));
   check_launch(launch_column_sum(dH1_.data(), B, w, gW(b1()), s));
 }
 

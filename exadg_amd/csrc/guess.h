@@ -105,7 +105,7 @@ private:
   int                        ws_B_      = 0;
   int                        adam_step_ = 0;
   DeviceBuffer<float>        params_, grads_, m_, v_;
-  DeviceBuffer<float>        X_, T_, H1_, H2_, Y_, dY_, dH1_, dH2_;
+  DeviceBuffer<float>        X_, T_, H1_, H2_, Y_, dY_, dH1_, dH2_, split_;
   DeviceBuffer<double>       partial_, loss_, moments_;
   DeviceBuffer<double>       e_, r_, q_;
   DeviceBuffer<unsigned int> bad_;

@@ -24,9 +24,13 @@ struct FeatureScales
   float inv_t    = 1.f;
 };
 
+// C(i,j) = alpha sum_k A(i,k) B(k,j) (+ C) (+ bias_j) (ReLU), A(i,k) = A[i a_i + k a_k],
+// B(k,j) = B[k b_k + j b_j]; with a workspace of >= s*M*N floats the K range may be split
+// over s blocks when M x N alone has too few tiles
 hipError_t launch_sgemm(int M, int N, int K, float const * A, int64_t a_i, int64_t a_k,
                         float const * B, int64_t b_k, int64_t b_j, float * C, int64_t ldc,
-                        float alpha, bool accumulate, float const * bias, bool relu, hipStream_t s);
+                        float alpha, bool accumulate, float const * bias, bool relu,
+                        float * workspace, int64_t workspace_floats, hipStream_t s);
 template<typename In>
 hipError_t launch_features(float * X, float * T, History<In> const & hist, In const * rhs,
                            In const * target, int history, int n_cell_dofs, int64_t cell0, int B,

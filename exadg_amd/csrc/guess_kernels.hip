@@ -16,31 +16,41 @@ constexpr int TM = 64, TN = 64, TK = 16;
 
 // C(i,j) = alpha * sum_k A(i,k) B(k,j) (+ C) (+ bias_j), optional ReLU.
 // A(i,k) = A[i*a_i + k*a_k], B(k,j) = B[k*b_k + j*b_j], C row-major with leading dim ldc.
+// With gridDim.z > 1 (split K), block z covers k in [z*kc, min(K, (z+1)*kc)) and writes its
+// raw partial product to C + z * M * ldc; splitk_reduce_kernel applies alpha/C/bias/ReLU.
 __global__ void __launch_bounds__(256)
   sgemm_kernel(int M, int N, int K, float const * __restrict__ A, int64_t a_i, int64_t a_k,
                float const * __restrict__ B, int64_t b_k, int64_t b_j, float * __restrict__ C,
-               int64_t ldc, float alpha, int accumulate, float const * __restrict__ bias, int relu)
+               int64_t ldc, float alpha, int accumulate, float const * __restrict__ bias, int relu,
+               int kc)
 {
   __shared__ float As[TK][TM + 1];
   __shared__ float Bs[TK][TN + 1];
   int const tid = threadIdx.x;
   int const tx = tid % 16, ty = tid / 16;
   int const i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
-  float     acc[4][4] = {};
-  for(int k0 = 0; k0 < K; k0 += TK)
+  int const kb = blockIdx.z * kc, ke = min(K, kb + kc);
+  bool const split = gridDim.z > 1;
+  if(split)
+  {
+    C += (int64_t)blockIdx.z * M * ldc;
+    alpha = 1.f, accumulate = 0, bias = nullptr, relu = 0;
+  }
+  float acc[4][4] = {};
+  for(int k0 = kb; k0 < ke; k0 += TK)
   {
     // 64x16 tiles of A and 16x64 of B: 4 elements per thread each
     for(int e = tid; e < TM * TK; e += 256)
     {
       int const r = e / TK, c = e % TK;
       int const i = i0 + r, k = k0 + c;
-      As[c][r]    = (i < M && k < K) ? A[i * a_i + k * a_k] : 0.f;
+      As[c][r]    = (i < M && k < ke) ? A[i * a_i + k * a_k] : 0.f;
     }
     for(int e = tid; e < TK * TN; e += 256)
     {
       int const r = e / TN, c = e % TN;
       int const k = k0 + r, j = j0 + c;
-      Bs[r][c]    = (k < K && j < N) ? B[k * b_k + j * b_j] : 0.f;
+      Bs[r][c]    = (k < ke && j < N) ? B[k * b_k + j * b_j] : 0.f;
     }
     __syncthreads();
 #pragma unroll
@@ -82,6 +92,30 @@ __global__ void __launch_bounds__(256)
         v = v > 0.f ? v : 0.f;
       C[i * ldc + j] = v;
     }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+  splitk_reduce_kernel(int M, int N, int splits, float const * __restrict__ part, float * C,
+                       int64_t ldc, float alpha, int accumulate, float const * __restrict__ bias,
+                       int relu)
+{
+  int64_t const total = (int64_t)M * N;
+  for(int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+      e += (int64_t)gridDim.x * blockDim.x)
+  {
+    int const i = (int)(e / N), j = (int)(e % N);
+    float     v = 0.f;
+    for(int z = 0; z < splits; ++z)
+      v += part[((int64_t)z * M + i) * N + j];
+    v *= alpha;
+    if(accumulate)
+      v += C[i * ldc + j];
+    if(bias)
+      v += bias[j];
+    if(relu)
+      v = v > 0.f ? v : 0.f;
+    C[i * ldc + j] = v;
   }
 }
 
@@ -460,10 +494,32 @@ inline int grid_for(int64_t n)
 
 hipError_t launch_sgemm(int M, int N, int K, float const * A, int64_t a_i, int64_t a_k,
                         float const * B, int64_t b_k, int64_t b_j, float * C, int64_t ldc,
-                        float alpha, bool accumulate, float const * bias, bool relu, hipStream_t s)
+                        float alpha, bool accumulate, float const * bias, bool relu,
+                        float * workspace, int64_t workspace_floats, hipStream_t s)
 {
-  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
-  hipLaunchKernelGGL(sgemm_kernel, grid, dim3(256), 0, s, M, N, K, A, a_i, a_k, B, b_k, b_j, C, ldc,
+  int const tiles = ((N + TN - 1) / TN) * ((M + TM - 1) / TM);
+  // split K when the output tiles alone cannot fill the chip (weight gradients: K = batch)
+  int splits = 1;
+  if(workspace != nullptr && tiles < 512 && K >= 8 * TK)
+  {
+    splits = std::min((512 + tiles - 1) / tiles, K / (4 * TK));
+    splits = (int)std::min<int64_t>(splits, workspace_floats / ((int64_t)M * N));
+    splits = std::max(splits, 1);
+  }
+  int const kc = splits > 1 ? ((K + splits - 1) / splits + TK - 1) / TK * TK : K;
+  splits       = (K + kc - 1) / kc;
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
+  if(splits == 1)
+  {
+    hipLaunchKernelGGL(sgemm_kernel, grid, dim3(256), 0, s, M, N, K, A, a_i, a_k, B, b_k, b_j, C, ldc,
+                       alpha, accumulate ? 1 : 0, bias, relu ? 1 : 0, kc);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(sgemm_kernel, grid, dim3(256), 0, s, M, N, K, A, a_i, a_k, B, b_k, b_j,
+                     workspace, (int64_t)N, 1.f, 0, nullptr, 0, kc);
+  int64_t const total = (int64_t)M * N;
+  int const     g     = (int)std::min<int64_t>((total + 255) / 256, 2048);
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(g), dim3(256), 0, s, M, N, splits, workspace, C, ldc,
                      alpha, accumulate ? 1 : 0, bias, relu ? 1 : 0);
   return hipGetLastError();
 }

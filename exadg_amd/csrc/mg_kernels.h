@@ -40,12 +40,17 @@ hipError_t launch_c_prolongate_add(Number * dst_dg, Number const * src_cg, BoxTo
 template<typename Number>
 hipError_t launch_c_restrict_add(Number * dst_cg, Number const * src_dg, BoxTopology const & t,
                                  Q1Topology const & q, hipStream_t s);
-// Q1 operator: dst = A src with per-cell 8x8 element matrices (one shared matrix when
-// `shared_matrix`); constrained rows are identity, constrained columns read as zero.
-// mode 0: vmult, 1: diagonal (src ignored)
+// Q1 operator as an assembled 27-point stencil (W: 27 planes of n_nodes weights, plane j for
+// the neighbour offset (j%3-1, (j/3)%3-1, j/9-1), periodic wrap; constrained rows identity)
 template<typename Number>
-hipError_t launch_q1_apply(Number * dst, Number const * src, Number const * cell_matrices,
-                           int shared_matrix, BoxTopology const & t, Q1Topology const & q,
-                           int mode, hipStream_t s);
+hipError_t launch_q1_stencil(Number * dst, Number const * src, Number const * W,
+                             Q1Topology const & q, hipStream_t s);
+// one Chebyshev-Jacobi step with the stencil application fused:
+// x_new = (1+f1) x - f1 x_old (if has_old) + f2 dinv (b - A x)
+template<typename Number>
+hipError_t launch_q1_chebyshev(Number * x_new, Number const * x, Number const * x_old,
+                               Number const * b, Number const * dinv, Number const * W,
+                               Q1Topology const & q, Number f1, Number f2, bool has_old,
+                               hipStream_t s);
 
 }  // namespace ExaDG

@@ -155,10 +155,45 @@ __global__ void __launch_bounds__(256)
   dst_cg[node] += acc;
 }
 
+// Q1 operator as an assembled 27-point stencil: W[j * n + node] couples node with its
+// neighbour at offset (j % 3 - 1, (j / 3) % 3 - 1, j / 9 - 1) (periodic wrap). Weights to
+// constrained or missing neighbours are zero and those reads are clamped in bounds; a
+// constrained node's row is the identity. One thread per node, W read plane by plane
+// (consecutive nodes, consecutive addresses).
+template<typename Number>
+__device__ __forceinline__ Number q1_stencil_row(Number const * __restrict__ W,
+                                                 Number const * __restrict__ src,
+                                                 Q1Topology const & q, int64_t node, int const * nd,
+                                                 int64_t n)
+{
+  int idx[3][3];
+#pragma unroll
+  for(int d = 0; d < 3; ++d)
+#pragma unroll
+    for(int o = 0; o < 3; ++o)
+    {
+      int c = nd[d] + o - 1;
+      if(q.periodic[d])
+        c = c < 0 ? c + q.nn[d] : (c >= q.nn[d] ? c - q.nn[d] : c);
+      else
+        c = c < 0 ? 0 : (c >= q.nn[d] ? q.nn[d] - 1 : c);
+      idx[d][o] = c;
+    }
+  Number acc = 0;
+#pragma unroll
+  for(int j = 0; j < 27; ++j)
+  {
+    int64_t const nb =
+      idx[0][j % 3] + (int64_t)q.nn[0] * (idx[1][(j / 3) % 3] + (int64_t)q.nn[1] * idx[2][j / 9]);
+    acc += W[j * n + node] * src[nb];
+  }
+  return acc;
+}
+
 template<typename Number>
 __global__ void __launch_bounds__(256)
-  q1_apply_kernel(Number * dst, Number const * src, Number const * mats, int shared, BoxTopology t,
-                  Q1Topology q, int mode, int64_t n)
+  q1_stencil_kernel(Number * __restrict__ dst, Number const * __restrict__ src,
+                    Number const * __restrict__ W, Q1Topology q, int64_t n)
 {
   int64_t const node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(node >= n)
@@ -168,42 +203,32 @@ __global__ void __launch_bounds__(256)
   int64_t const r = node / q.nn[0];
   nd[1]           = (int)(r % q.nn[1]);
   nd[2]           = (int)(r / q.nn[1]);
-  if(q1_constrained(q, nd))
-  {  // identity row
-    dst[node] = mode == 0 ? src[node] : Number(1);
+  dst[node]       = q1_stencil_row(W, src, q, node, nd, n);
+}
+
+// one Chebyshev step on the Q1 level with the operator application fused in:
+//   x_new = (1 + f1) x - f1 x_old + f2 D^{-1} (b - A x)     (variant 2)
+//   x_new = (1 + f1) x + f2 D^{-1} (b - A x)                (variant 1)
+template<typename Number>
+__global__ void __launch_bounds__(256)
+  q1_chebyshev_kernel(Number * __restrict__ x_new, Number const * __restrict__ x,
+                      Number const * __restrict__ x_old, Number const * __restrict__ b,
+                      Number const * __restrict__ dinv, Number const * __restrict__ W,
+                      Q1Topology q, Number f1, Number f2, int variant, int64_t n)
+{
+  int64_t const node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(node >= n)
     return;
-  }
-  Number acc = 0;
-  for(int v = 0; v < 8; ++v)
-  {
-    int  cc[3];
-    bool ok = true;
-#pragma unroll
-    for(int d = 0; d < 3; ++d)
-    {
-      int c = nd[d] - ((v >> d) & 1);
-      if(c < 0)
-        c = q.periodic[d] ? c + q.nc[d] : -1;
-      if(c < 0 || c >= q.nc[d])
-        ok = false;
-      cc[d] = c;
-    }
-    if(!ok)
-      continue;
-    int64_t const  cell = cell_id(t, cc);
-    Number const * K    = mats + (shared ? 0 : cell * 64) + v * 8;
-    if(mode == 1)
-      acc += K[v];
-    else
-      for(int m = 0; m < 8; ++m)
-      {
-        int nm[3];
-        q1_vertex_node(q, cc, m, nm);
-        if(!q1_constrained(q, nm))
-          acc += K[m] * src[q1_node_id(q, nm)];
-      }
-  }
-  dst[node] = acc;
+  int nd[3];
+  nd[0]           = (int)(node % q.nn[0]);
+  int64_t const r = node / q.nn[0];
+  nd[1]           = (int)(r % q.nn[1]);
+  nd[2]           = (int)(r / q.nn[1]);
+  Number const t  = q1_stencil_row(W, x, q, node, nd, n);
+  Number       v  = (Number(1) + f1) * x[node] + f2 * dinv[node] * (b[node] - t);
+  if(variant == 2)
+    v -= f1 * x_old[node];
+  x_new[node] = v;
 }
 
 inline int blocks_for(int64_t n) { return (int)((n + 255) / 256); }
@@ -252,12 +277,24 @@ hipError_t launch_c_restrict_add(Number * dst_cg, Number const * src_dg, BoxTopo
 }
 
 template<typename Number>
-hipError_t launch_q1_apply(Number * dst, Number const * src, Number const * mats, int shared,
-                           BoxTopology const & t, Q1Topology const & q, int mode, hipStream_t s)
+hipError_t launch_q1_stencil(Number * dst, Number const * src, Number const * W,
+                             Q1Topology const & q, hipStream_t s)
 {
   int64_t const n = (int64_t)q.nn[0] * q.nn[1] * q.nn[2];
-  hipLaunchKernelGGL((q1_apply_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, dst, src, mats,
-                     shared, t, q, mode, n);
+  hipLaunchKernelGGL((q1_stencil_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, dst, src, W,
+                     q, n);
+  return hipGetLastError();
+}
+
+template<typename Number>
+hipError_t launch_q1_chebyshev(Number * x_new, Number const * x, Number const * x_old,
+                               Number const * b, Number const * dinv, Number const * W,
+                               Q1Topology const & q, Number f1, Number f2, bool has_old,
+                               hipStream_t s)
+{
+  int64_t const n = (int64_t)q.nn[0] * q.nn[1] * q.nn[2];
+  hipLaunchKernelGGL((q1_chebyshev_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, x_new, x,
+                     x_old, b, dinv, W, q, f1, f2, has_old ? 2 : 1, n);
   return hipGetLastError();
 }
 
@@ -270,8 +307,10 @@ hipError_t launch_q1_apply(Number * dst, Number const * src, Number const * mats
                                                  Q1Topology const &, hipStream_t);                 \
   template hipError_t launch_c_restrict_add<T>(T *, T const *, BoxTopology const &,                \
                                                Q1Topology const &, hipStream_t);                   \
-  template hipError_t launch_q1_apply<T>(T *, T const *, T const *, int, BoxTopology const &,      \
-                                         Q1Topology const &, int, hipStream_t);
+  template hipError_t launch_q1_stencil<T>(T *, T const *, T const *, Q1Topology const &,          \
+                                           hipStream_t);                                           \
+  template hipError_t launch_q1_chebyshev<T>(T *, T const *, T const *, T const *, T const *,      \
+                                             T const *, Q1Topology const &, T, T, bool, hipStream_t);
 EXADG_INST(double)
 EXADG_INST(float)
 #undef EXADG_INST

@@ -181,11 +181,11 @@ Q1Operator<Number>::Q1Operator(StructuredGrid const & grid) : topo_(grid.topolog
       q_.dirichlet[2 * d + s] = (!topo_.periodic[d] && topo_.bc[2 * d + s] == EXADG_BC_DIRICHLET);
   }
   n_nodes_ = (int64_t)q_.nn[0] * q_.nn[1] * q_.nn[2];
-  shared_  = grid.affine() ? 1 : 0;
+  bool const shared = grid.affine();
   // element stiffness matrices int grad(phi_a) . grad(phi_b) with QGauss(2)
   std::vector<double> gp, gw;
   gauss_points(2, gp, gw);
-  int64_t const nmat = shared_ ? 1 : grid.n_cells();
+  int64_t const nmat = shared ? 1 : grid.n_cells();
   mats_host_.assign((size_t)nmat * 64, 0.0);
   parallel_range(nmat, [&](int64_t cb, int64_t ce) {
     std::vector<double> x(3 * 8), J(9 * 8);
@@ -237,23 +237,88 @@ Q1Operator<Number>::Q1Operator(StructuredGrid const & grid) : topo_(grid.topolog
       }
     }
   });
-  mats_.upload(std::vector<Number>(mats_host_.begin(), mats_host_.end()));
+  // assemble the 27-point stencil: node-centric gather over the (up to) 8 cells sharing the
+  // node; row v of a cell's matrix couples its vertex v with the vertices m at offset
+  // bits(m) - bits(v). Constrained neighbours are dropped (their columns read as zero),
+  // constrained rows become identity rows.
+  std::vector<char> const mask = constrained_mask();
+  int64_t const           nn   = n_nodes_;
+  std::vector<double>     W((size_t)27 * nn, 0.0), D((size_t)nn, 0.0);
+  parallel_range(nn, [&](int64_t nb, int64_t ne) {
+    for(int64_t node = nb; node < ne; ++node)
+    {
+      if(mask[node])
+      {
+        W[(size_t)13 * nn + node] = 1.0;
+        D[node]                   = 1.0;
+        continue;
+      }
+      int const nd[3] = {(int)(node % q_.nn[0]), (int)((node / q_.nn[0]) % q_.nn[1]),
+                         (int)(node / ((int64_t)q_.nn[0] * q_.nn[1]))};
+      for(int v = 0; v < 8; ++v)
+      {
+        int  cc[3];
+        bool ok = true;
+        for(int d = 0; d < 3; ++d)
+        {
+          int c = nd[d] - ((v >> d) & 1);
+          if(c < 0)
+            c = q_.periodic[d] ? c + q_.nc[d] : -1;
+          if(c < 0 || c >= q_.nc[d])
+            ok = false;
+          cc[d] = c;
+        }
+        if(!ok)
+          continue;
+        int64_t const  cell = cell_id(topo_, cc);
+        double const * K    = &mats_host_[(grid.affine() ? 0 : (size_t)cell * 64) + v * 8];
+        D[node] += K[v];
+        for(int m = 0; m < 8; ++m)
+        {
+          int nm[3], off[3];
+          for(int d = 0; d < 3; ++d)
+          {
+            int const c = cc[d] + ((m >> d) & 1);
+            nm[d]       = (q_.periodic[d] && c == q_.nc[d]) ? 0 : c;
+            off[d]      = ((m >> d) & 1) - ((v >> d) & 1);
+          }
+          int64_t const id = nm[0] + (int64_t)q_.nn[0] * (nm[1] + (int64_t)q_.nn[1] * nm[2]);
+          if(mask[id])
+            continue;
+          int const j = (off[0] + 1) + 3 * (off[1] + 1) + 9 * (off[2] + 1);
+          W[(size_t)j * nn + node] += K[m];
+        }
+      }
+    }
+  });
+  stencil_.upload(std::vector<Number>(W.begin(), W.end()));
+  diag_.upload(std::vector<Number>(D.begin(), D.end()));
 }
 
 template<typename Number>
 void Q1Operator<Number>::vmult(Number * dst, Number const * src, hipStream_t s) const
 {
-  hipError_t e = launch_q1_apply<Number>(dst, src, mats_.data(), shared_, topo_, q_, 0, s);
+  hipError_t e = launch_q1_stencil<Number>(dst, src, stencil_.data(), q_, s);
   if(e != hipSuccess)
     EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
 }
 
 template<typename Number>
-void Q1Operator<Number>::inverse_diagonal(Number * d, hipStream_t s) const
+bool Q1Operator<Number>::chebyshev_step(Number * x_new, Number const * x, Number const * x_old,
+                                        Number const * b, Number const * dinv, Number f1, Number f2,
+                                        bool has_old, hipStream_t s) const
 {
-  hipError_t e = launch_q1_apply<Number>(d, nullptr, mats_.data(), shared_, topo_, q_, 1, s);
+  hipError_t e =
+    launch_q1_chebyshev<Number>(x_new, x, x_old, b, dinv, stencil_.data(), q_, f1, f2, has_old, s);
   if(e != hipSuccess)
     EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  return true;
+}
+
+template<typename Number>
+void Q1Operator<Number>::inverse_diagonal(Number * d, hipStream_t s) const
+{
+  vec::copy(d, diag_.data(), n_nodes_, s);
   Poisson::invert_diagonal_device<Number>(d, n_nodes_, s);
 }
 
@@ -375,12 +440,10 @@ void ChebyshevSmoother<Number>::vmult(Number *& x, Number *& x_alt, Number const
   double rhok = delta / theta, sigma = theta / delta;
   for(int i = 1; i < degree; ++i)
   {
-    op_->vmult(t_.data(), x, s);
     double const rhokp = 1. / (2. * sigma - rhok);
     double const f1 = rhokp * rhok, f2 = 2. * rhokp / delta;
     rhok            = rhokp;
-    vec::chebyshev_update<Number>(x_alt, x, x_alt, b, t_.data(), inv_diag_.data(), Number(f1),
-                                  Number(f2), false, i > 1, n, s);
+    apply_step(x_alt, x, x_alt, b, Number(f1), Number(f2), i > 1, s);
     std::swap(x, x_alt);
   }
 }
@@ -389,24 +452,31 @@ template<typename Number>
 void ChebyshevSmoother<Number>::step(Number *& x, Number *& x_alt, Number const * b,
                                      hipStream_t s) const
 {
-  int64_t const n = op_->n();
-  op_->vmult(t_.data(), x, s);
-  vec::chebyshev_update<Number>(x_alt, x, x_alt, b, t_.data(), inv_diag_.data(), Number(0),
-                                Number(1. / theta), false, false, n, s);
+  apply_step(x_alt, x, x_alt, b, Number(0), Number(1. / theta), false, s);
   std::swap(x, x_alt);
   if(degree < 2 || std::fabs(delta) < 1e-40)
     return;
   double rhok = delta / theta, sigma = theta / delta;
   for(int i = 1; i < degree; ++i)
   {
-    op_->vmult(t_.data(), x, s);
     double const rhokp = 1. / (2. * sigma - rhok);
     double const f1 = rhokp * rhok, f2 = 2. * rhokp / delta;
     rhok            = rhokp;
-    vec::chebyshev_update<Number>(x_alt, x, x_alt, b, t_.data(), inv_diag_.data(), Number(f1),
-                                  Number(f2), false, true, n, s);
+    apply_step(x_alt, x, x_alt, b, Number(f1), Number(f2), true, s);
     std::swap(x, x_alt);
   }
+}
+
+template<typename Number>
+void ChebyshevSmoother<Number>::apply_step(Number * x_new, Number const * x, Number const * x_old,
+                                           Number const * b, Number f1, Number f2, bool has_old,
+                                           hipStream_t s) const
+{
+  if(op_->chebyshev_step(x_new, x, x_old, b, inv_diag_.data(), f1, f2, has_old, s))
+    return;
+  op_->vmult(t_.data(), x, s);
+  vec::chebyshev_update<Number>(x_new, x, x_old, b, t_.data(), inv_diag_.data(), f1, f2, false,
+                                has_old, op_->n(), s);
 }
 
 // ---------------------------------------------------------------- multigrid
@@ -479,8 +549,26 @@ MultigridPreconditioner<Number>::MultigridPreconditioner(std::shared_ptr<Structu
 }
 
 template<typename Number>
+MultigridPreconditioner<Number>::~MultigridPreconditioner()
+{
+  clear_graphs();
+  if(capture_stream_)
+    (void)hipStreamDestroy(capture_stream_);
+}
+
+template<typename Number>
+void MultigridPreconditioner<Number>::clear_graphs()
+{
+  for(Graph & g : graphs_)
+    (void)hipGraphExecDestroy(g.exec);
+  graphs_.clear();
+  eager_calls_ = 0;
+}
+
+template<typename Number>
 void MultigridPreconditioner<Number>::update(hipStream_t s)
 {
+  clear_graphs();  // the captured kernels hold the Chebyshev coefficients as arguments
   for(int l = 1; l < n_levels(); ++l)
     levels_[l].smoother.setup(*levels_[l].op, data_.smoother_degree, data_.smoothing_range,
                               data_.eig_cg_iterations, s);
@@ -569,7 +657,7 @@ void MultigridPreconditioner<Number>::v_cycle(int l, hipStream_t s)
 }
 
 template<typename Number>
-void MultigridPreconditioner<Number>::vmult(double * dst, double const * src, hipStream_t s)
+void MultigridPreconditioner<Number>::vmult_eager(double * dst, double const * src, hipStream_t s)
 {
   int const     top = n_levels() - 1;
   int64_t const n   = levels_[top].op->n();
@@ -584,6 +672,55 @@ void MultigridPreconditioner<Number>::vmult(double * dst, double const * src, hi
     vec::copy(dst, levels_[top].sol, n, s);
   else
     vec::convert<double, Number>(dst, levels_[top].sol, n, s);
+}
+
+template<typename Number>
+void MultigridPreconditioner<Number>::vmult(double * dst, double const * src, hipStream_t s)
+{
+  // The first call runs eagerly (kernel attributes get set outside any capture). Later calls
+  // replay a graph of the whole V-cycle: its ~10^2 small launches on the coarse levels cost
+  // more in launch latency than in execution. Every buffer a cycle reads is written earlier in
+  // the same cycle, so each replay computes the same function of src. The capture records
+  // only kernels and device copies (no memset nodes) on a private stream; the replay runs on s.
+  if(!uses_graph() || eager_calls_ == 0)
+  {
+    vmult_eager(dst, src, s);
+    ++eager_calls_;
+    return;
+  }
+  Graph const * g = nullptr;
+  for(Graph const & c : graphs_)
+    if(c.dst == dst && c.src == src)
+      g = &c;
+  if(g == nullptr)
+  {
+    if(graphs_.size() >= 8)
+      clear_graphs(), eager_calls_ = 1;
+    if(capture_stream_ == nullptr)
+      HIP_CHECK(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal));
+    try
+    {
+      vmult_eager(dst, src, capture_stream_);
+    }
+    catch(...)
+    {
+      (void)hipStreamEndCapture(capture_stream_, &graph);
+      if(graph)
+        (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(capture_stream_, &graph));
+    hipGraphExec_t exec = nullptr;
+    hipError_t const e  = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if(e != hipSuccess)
+      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+    graphs_.push_back({dst, src, exec});
+    g = &graphs_.back();
+  }
+  HIP_CHECK(hipGraphLaunch(g->exec, s));
 }
 
 std::unique_ptr<PreconditionerBase> create_multigrid(std::shared_ptr<StructuredGrid const> grid,

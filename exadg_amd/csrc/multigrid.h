@@ -59,6 +59,9 @@ struct MultigridData
   double           coarse_rel_tol  = 1e-3;
   int              coarse_max_iter = 10000;
   bool             operator_is_singular = false;
+  // replay the V-cycle as a HIP graph (only when it has no host synchronization, i.e. with
+  // the Chebyshev coarse solver); one graph per (dst, src) pair, rebuilt by update()
+  bool capture_graph = false;
 };
 
 // work vectors of a PCG on one level (reused across solves)
@@ -86,6 +89,14 @@ public:
   virtual bool    continuous() const { return false; }
   // mask of constrained entries (1 constrained, 0 free) on the host; empty if none
   virtual std::vector<char> constrained_mask() const { return {}; }
+  // x_new = (1+f1) x - f1 x_old (if has_old) + f2 dinv (b - A x) in one pass; false if this
+  // operator has no fused form (the caller then applies A and updates separately)
+  virtual bool chebyshev_step(Number * /*x_new*/, Number const * /*x*/, Number const * /*x_old*/,
+                              Number const * /*b*/, Number const * /*dinv*/, Number /*f1*/,
+                              Number /*f2*/, bool /*has_old*/, hipStream_t /*s*/) const
+  {
+    return false;
+  }
 };
 
 // continuous Q1 operator with strong Dirichlet constraints on the structured box
@@ -99,6 +110,9 @@ public:
   void    inverse_diagonal(Number * d, hipStream_t s) const override;
   bool    continuous() const override { return true; }
   std::vector<char>  constrained_mask() const override;
+  bool chebyshev_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
+                      Number const * dinv, Number f1, Number f2, bool has_old,
+                      hipStream_t s) const override;
   Q1Topology const & q1_topology() const { return q_; }
   std::vector<double> const & element_matrices() const { return mats_host_; }
 
@@ -106,9 +120,9 @@ private:
   BoxTopology          topo_;
   Q1Topology           q_;
   int64_t              n_nodes_;
-  int                  shared_;
-  std::vector<double>  mats_host_;
-  DeviceBuffer<Number> mats_;
+  std::vector<double>  mats_host_;  // 8x8 element matrices (one shared one if affine)
+  DeviceBuffer<Number> stencil_;    // assembled 27-point stencil, 27 planes of n_nodes
+  DeviceBuffer<Number> diag_;       // sum over cells of K_vv (1 on constrained rows)
 };
 
 // Chebyshev polynomial smoother with point-Jacobi inner preconditioner (deal.II
@@ -129,6 +143,10 @@ public:
   int    degree = 0, cg_iterations = 0;
 
 private:
+  // x_new = (1+f1) x - f1 x_old (has_old) + f2 D^{-1} (b - A x), fused where the operator can
+  void apply_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
+                  Number f1, Number f2, bool has_old, hipStream_t s) const;
+
   LevelOperator<Number> const * op_ = nullptr;
   DeviceVector<Number>          inv_diag_;
   mutable DeviceVector<Number>  t_;
@@ -140,6 +158,14 @@ class MultigridPreconditioner : public PreconditionerBase
 public:
   MultigridPreconditioner(std::shared_ptr<StructuredGrid const> grid, MultigridData const & data,
                           double ip_factor, hipStream_t s);
+  ~MultigridPreconditioner() override;
+  MultigridPreconditioner(MultigridPreconditioner const &) = delete;
+  MultigridPreconditioner & operator=(MultigridPreconditioner const &) = delete;
+  // true if vmult replays captured graphs
+  bool uses_graph() const
+  {
+    return data_.capture_graph && data_.coarse_solver == CoarseSolver::chebyshev;
+  }
   void vmult(double * dst, double const * src, hipStream_t s) override;
   void update(hipStream_t s) override;
 
@@ -166,6 +192,18 @@ private:
   };
   void v_cycle(int l, hipStream_t s);
   void coarse_solve(hipStream_t s);
+  void vmult_eager(double * dst, double const * src, hipStream_t s);
+  void clear_graphs();
+
+  struct Graph
+  {
+    double *        dst;
+    double const *  src;
+    hipGraphExec_t  exec;
+  };
+  std::vector<Graph> graphs_;
+  hipStream_t        capture_stream_ = nullptr;
+  int                eager_calls_    = 0;
 
   std::shared_ptr<StructuredGrid const> grid_;
   MultigridData                         data_;

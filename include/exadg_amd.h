@@ -140,6 +140,9 @@ typedef struct exadg_mg_desc
   double  coarse_rel_tol;    /* CoarseGridData solver_data.rel_tol (1e-3)             */
   int32_t coarse_max_iter;
   int32_t operator_is_singular;
+  int32_t capture_graph;     /* replay the V-cycle as a HIP graph when it has no host
+                                synchronization (Chebyshev coarse solver); default 0: on
+                                MI355X the replay measured no faster than eager launches */
 } exadg_mg_desc;
 
 /* Defaults of the reference for degree k: PSequenceType::Bisect (p -> max(p/2,1),

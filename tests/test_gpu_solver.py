@@ -87,6 +87,55 @@ def test_vcycle_fp32_close_to_oracle(case, torch_cuda, product):
     assert err <= 1e-4
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", ["curved_k4", "periodic_singular_k4"])
+def test_vcycle_graph_replays_match_eager(case, precision, torch_cuda, product):
+    """With the Chebyshev coarse solver the V-cycle is replayed as a HIP graph from the second
+    call on. Every replay (with eager work and synchronizations in between, and a second
+    (dst, src) pair that needs its own graph) equals the eager V-cycle."""
+    torch = torch_cuda
+    prec_t = product.FP64 if precision == "fp64" else product.FP32
+    if case == "curved_k4":
+        mesh = product.mesh_desc(3, deformation=0.1, mapping_degree=3)
+        bc, singular = product.bc_desc((D, N, D, D, D, D)), False
+    else:
+        mesh = product.mesh_desc(4, periodic=(1, 1, 0))
+        bc, singular = product.bc_desc((N,) * 6), True
+    op = product.LaplaceOperator(mesh, bc, 4)
+    descs = [product.mg_desc(4, precision=prec_t, coarse_solver=product.COARSE_CHEBYSHEV,
+                             singular=singular, capture_graph=g) for g in (True, False)]
+    pg = product.Preconditioner(op, product.PRECOND_MULTIGRID, descs[0])
+    pe = product.Preconditioner(op, product.PRECOND_MULTIGRID, descs[1])
+    r = torch.from_numpy(oracle.random_vector(op.n)).cuda()
+    r2 = torch.from_numpy(oracle.random_vector(op.n, seed=7)).cuda()
+    ze, ze2 = torch.empty_like(r), torch.empty_like(r)
+    pe.vmult(ze, r)
+    pe.vmult(ze2, r2)
+    torch.cuda.synchronize()
+    zg, zg2, y = torch.empty_like(r), torch.empty_like(r), torch.empty_like(r)
+    scale = float(ze.abs().max())
+    for rep in range(5):
+        zg.fill_(float("nan"))
+        pg.vmult(zg, r)
+        op.vmult(y, r)  # eager work between replays
+        torch.cuda.synchronize()
+        assert float((zg - ze).abs().max()) <= 1e-14 * scale, rep
+        if rep >= 2:
+            zg2.zero_()
+            pg.vmult(zg2, r2)
+            torch.cuda.synchronize()
+            assert float((zg2 - ze2).abs().max()) <= 1e-14 * float(ze2.abs().max()), rep
+    # a solve through the graph takes the eager solve's iterations
+    b = torch.zeros_like(r)
+    op.vmult(b, r)
+    if singular:
+        product.subtract_mean(b)
+    xg, xe = torch.zeros_like(b), torch.zeros_like(b)
+    ig = product.cg_solve(op, b, xg, pg, rel_tol=1e-10, rel_mode=1)
+    ie = product.cg_solve(op, b, xe, pe, rel_tol=1e-10, rel_mode=1)
+    assert ig["iterations"] == ie["iterations"] and ig["converged"]
+
+
 def _sine(product, k, curvilinear):
     dfm = 0.15 if curvilinear else 0.0
     mp = product.mesh_desc(8, -1.0, 1.0, deformation=dfm, frequency=2, mapping_degree=3)
