@@ -47,6 +47,46 @@ def cpu_baseline(degree, n_cells_1d, target_s=10.0):
                       f"{len(times)} calls"}
 
 
+def curved_bytes(k, n_cells_1d, s=8):
+    """Algorithmic bytes of the curved vmult: src + dst, the per-point metric (6 per Gauss
+    point), the per-face-point data (7 per point) and tau per face."""
+    n = k + 1
+    nc = n_cells_1d ** 3
+    n_faces = 3 * n_cells_1d ** 2 * (n_cells_1d + 1)
+    return 2 * s * nc * n ** 3 + 6 * s * n ** 3 * nc + 7 * s * n ** 2 * n_faces + 8 * n_faces
+
+
+def run_sweep(ex, torch, args):
+    """Config 2, curved part: [0,1]^3 deformed by x_d += 0.1 prod sin(pi x_e), MappingQ(k),
+    FP64 vmult, seeded uniform[-1,1] input, median of HIP-event timings."""
+    res = []
+    for k, m in ((2, 96), (3, 72), (5, 48), (7, 36)):
+        op = ex.LaplaceOperator(ex.mesh_desc(m, deformation=0.1, frequency=1, mapping_degree=k),
+                                ex.bc_desc(), k, ex.FP64)
+        gen = torch.Generator(device="cuda").manual_seed(k)
+        x = torch.rand(op.n, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+        y = torch.empty_like(x)
+        for _ in range(3):
+            op.vmult(y, x)
+        stream = torch.cuda.current_stream()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(20)]
+        for a, b in ev:
+            a.record(stream)
+            op.vmult(y, x)
+            b.record(stream)
+        torch.cuda.synchronize()
+        t = statistics.median(a.elapsed_time(b) for a, b in ev) * 1e-3
+        nb = curved_bytes(k, m)
+        res.append({"degree": k, "cells": f"{m}^3", "n_dofs": op.n,
+                    "gdofs_per_s": round(op.n / t / 1e9, 2), "ms": round(t * 1e3, 4),
+                    "algorithmic_GBps": round(nb / t / 1e9, 1),
+                    "frac_of_hbm_peak": round(nb / t / 1e9 / HBM_PEAK_GBS, 4)})
+        del op, x, y
+        torch.cuda.empty_cache()
+    return res
+
+
 def run_guess(ex, torch, args):
     """Config 4: pressure sequence on a periodic-x/y walled-z box; train on the first steps,
     A/B on the rest. FP32-MG arms: zero / extrapolated / learned guess; FP64-MG arms (the plain
@@ -104,6 +144,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true",
                     help="also time the CG + p-multigrid solve (config 1 and curved k=5 48^3)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also time the curved-mesh vmult at degrees 2, 3, 5, 7 (~24M DoF each)")
     ap.add_argument("--guess", action="store_true",
                     help="also run the learned-initial-guess pressure sequence (config 4)")
     ap.add_argument("--guess-cells", type=int, default=32)
@@ -186,7 +228,7 @@ def main():
     }
     if args.solve:
         from exadg_amd import solve_bench
-        del x, y
+        x = y = None
         torch.cuda.empty_cache()
         sol = {}
         sol["config1_cart_k3_8c"] = solve_bench.compare_arms(
@@ -196,8 +238,12 @@ def main():
             exadg_amd, torch, exadg_amd.mesh_desc(48, deformation=0.1, mapping_degree=5),
             exadg_amd.bc_desc(), 5, [5, 3, 2, 1], 3.141592653589793, exadg_amd.COARSE_CHEBYSHEV)
         out["solve"] = sol
+    if args.sweep:
+        x = y = None
+        torch.cuda.empty_cache()
+        out["curved_sweep"] = run_sweep(exadg_amd, torch, args)
     if args.guess:
-        del x, y
+        x = y = None
         torch.cuda.empty_cache()
         out["learned_guess"] = run_guess(exadg_amd, torch, args)
     if not args.no_cpu:

@@ -12,7 +12,10 @@ enum class ApplyMode : int
 {
   vmult     = 0,  // dst = A src
   vmult_add = 1,  // dst += A src
-  diagonal  = 2   // dst[cell, i] = (A_KK)_{ii}: unit-vector probing of the cell-local block
+  diagonal  = 2,  // dst[cell, i] = (A_KK)_{ii}: unit-vector probing of the cell-local block
+  // Chebyshev-Jacobi step fused into the epilogue (src = x, dst = x_new):
+  // x_new = (1 + f1) x - f1 x_old (if has_old) + f2 dinv (b - A x); x_old may alias dst
+  chebyshev = 3
 };
 
 // Table layout (Number): general kernel: [S n*n | Dq n*n | d0 n | d1 n];
@@ -30,6 +33,13 @@ struct SipgArgs
   int64_t        n_cells;
   int            probe;      // diagonal mode: local index of the unit vector
   int            tile_rows;  // traversal: y-rows per tile (one tile per XCD)
+  // chebyshev mode only
+  Number const * x_old   = nullptr;
+  Number const * rhs     = nullptr;
+  Number const * dinv    = nullptr;
+  Number         f1      = 0;
+  Number         f2      = 0;
+  int            has_old = 0;
 };
 
 // degree 1..8; mode as above. Returns hipError of the launch.

@@ -496,6 +496,15 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
     else if constexpr(mode == ApplyMode::vmult_add)
       for(int e = lt; e < N3; e += N2)
         a.dst[cell * N3 + e] += A[e];
+    else if constexpr(mode == ApplyMode::chebyshev)
+      for(int e = lt; e < N3; e += N2)
+      {
+        int64_t const g = cell * N3 + e;
+        Number        v = (Number(1) + a.f1) * a.src[g] + a.f2 * a.dinv[g] * (a.rhs[g] - A[e]);
+        if(a.has_old)
+          v -= a.f1 * a.x_old[g];
+        a.dst[g] = v;
+      }
     else
       for(int e = lt; e < N3; e += N2)
         a.dst[cell * N3 + e] = A[e];
@@ -598,15 +607,18 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   int const     lt     = threadIdx.x - slot * N2;
   int const     pos0   = (int)xcd_logical_block(blockIdx.x, gridDim.x) * CPB;
   bool const    active = slot < CPB && pos0 + slot < (int)a.n_cells;
-  Number *      U      = smem + (active ? slot : 0) * 3 * L::PC;
+  // two LDS buffers per cell: U (the cell's values; from the Y phase on it holds B, whose
+  // lines each lane writes exactly where it read its own U line) and A
+  Number *      U      = smem + (active ? slot : 0) * 2 * L::PC;
   Number *      Abuf   = U + L::PC;
-  Number *      Bbuf   = Abuf + L::PC;
+  Number *      Bbuf   = U;
   int const     p = lt % N, q = lt / N;
   // LDS line bases of this lane for the x, y and z sweeps
   int const     bx = L::PX * p + L::PY * q, by = p + L::PY * q, bz = p + L::PX * q;
 
   int    type[3][2];
   Number tau[3][2], wnb[3][2], dwnb[3][2];
+  Number own[N];  // this lane's z-line of the cell (lt + m N2), used by the Z phase
   int    cell       = 0;
   bool   in_block_l = false, in_block_r = false;
   if(active)
@@ -637,7 +649,10 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
     {
 #pragma unroll
       for(int m = 0; m < N; ++m)
-        U[bz + L::PY * m] = (lt + m * N2 == a.probe) ? Number(1) : Number(0);
+      {
+        own[m]            = (lt + m * N2 == a.probe) ? Number(1) : Number(0);
+        U[bz + L::PY * m] = own[m];
+      }
       // neighbours contribute zero, except a cell that is its own periodic neighbour
 #pragma unroll
       for(int d = 0; d < 3; ++d)
@@ -660,7 +675,6 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
     {
       // all global loads up front: own cell and the six neighbour lines of this lane
       Number const * own_p = a.src + (size_t)cell * N3 + lt;
-      Number         own[N];
 #pragma unroll
       for(int m = 0; m < N; ++m)
         own[m] = own_p[m * N2];
@@ -718,13 +732,13 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
       Number w[N];
       if(in_block_l)
       {
-        load_line<N>(U - 3 * L::PC + bx, 1, w);
+        load_line<N>(U - 2 * L::PC + bx, 1, w);
         wnb[0][0]  = w[N - 1];
         dwnb[0][0] = dot_line<N>(tx + 2 * N2 + N, w);
       }
       if(in_block_r)
       {
-        load_line<N>(U + 3 * L::PC + bx, 1, w);
+        load_line<N>(U + 2 * L::PC + bx, 1, w);
         wnb[0][1]  = w[0];
         dwnb[0][1] = dot_line<N>(tx + 2 * N2, w);
       }
@@ -768,9 +782,8 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   Number outp[N];
   if(active)
   {
-    Number u[N], o[N], v[N];
-    load_line<N>(U + bz, L::PY, u);
-    line_laplacian<N>(tz, u, o, type[2], tau[2], wnb[2], dwnb[2]);
+    Number o[N], v[N];
+    line_laplacian<N>(tz, own, o, type[2], tau[2], wnb[2], dwnb[2]);
     load_line<N>(Abuf + bz, L::PY, v);
     mat_apply<N>(tz, v, outp);
     store_line<N>(Bbuf + bz, L::PY, o);
@@ -795,27 +808,40 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   }
   __syncthreads();
   // out = out_partial + B along the z-lines
-  if(active)
+  if constexpr(mode == ApplyMode::diagonal)
   {
-    Number v[N];
-    load_line<N>(Bbuf + bz, L::PY, v);
-    if constexpr(mode == ApplyMode::diagonal)
+    if(active)
     {
+      Number v[N];
+      load_line<N>(Bbuf + bz, L::PY, v);
 #pragma unroll
       for(int k = 0; k < N; ++k)
         if(lt + k * N2 == a.probe)
           a.dst[(size_t)cell * N3 + a.probe] = outp[k] + v[k];
     }
-    else
-    {
-      Number * out = a.dst + (size_t)cell * N3 + lt;
+  }
+  else if(active)
+  {
+    // direct per-lane stores: lane (p, q) owns the z-line of DoFs lt + k N2 (a workgroup-wide
+    // LDS-staged store of the block's contiguous cells measured 10% slower)
+    Number v[N];
+    load_line<N>(Bbuf + bz, L::PY, v);
+    size_t const g0 = (size_t)cell * N3 + lt;
 #pragma unroll
-      for(int k = 0; k < N; ++k)
+    for(int k = 0; k < N; ++k)
+    {
+      Number const ax = outp[k] + v[k];
+      size_t const g  = g0 + k * N2;
+      if constexpr(mode == ApplyMode::vmult)
+        a.dst[g] = ax;
+      else if constexpr(mode == ApplyMode::vmult_add)
+        a.dst[g] += ax;
+      else
       {
-        if constexpr(mode == ApplyMode::vmult_add)
-          out[k * N2] += outp[k] + v[k];
-        else
-          out[k * N2] = outp[k] + v[k];
+        Number r = (Number(1) + a.f1) * own[k] + a.f2 * a.dinv[g] * (a.rhs[g] - ax);
+        if(a.has_old)
+          r -= a.f1 * a.x_old[g];
+        a.dst[g] = r;
       }
     }
   }
@@ -845,7 +871,7 @@ hipError_t launch_cartesian_n(SipgArgs<Number> const & a, hipStream_t stream)
 {
   constexpr int CPB    = cells_per_block(N);
   int const     blocks = (int)((a.n_cells + CPB - 1) / CPB);
-  size_t const  lds    = sizeof(Number) * CPB * 3 * LdsLayout<N>::PC;
+  size_t const  lds    = sizeof(Number) * CPB * 2 * LdsLayout<N>::PC;
   hipLaunchKernelGGL((sipg_cartesian_kernel<N, Number, mode>), dim3(blocks), dim3(256), lds, stream,
                      a);
   return hipGetLastError();
@@ -895,6 +921,7 @@ hipError_t launch_sipg_general(int degree, ApplyMode mode, SipgArgs<Number> cons
     case ApplyMode::vmult: return dispatch_general<Number, ApplyMode::vmult>(degree, a, s);
     case ApplyMode::vmult_add: return dispatch_general<Number, ApplyMode::vmult_add>(degree, a, s);
     case ApplyMode::diagonal: return dispatch_general<Number, ApplyMode::diagonal>(degree, a, s);
+    case ApplyMode::chebyshev: return dispatch_general<Number, ApplyMode::chebyshev>(degree, a, s);
   }
   return hipErrorInvalidValue;
 }
@@ -908,6 +935,8 @@ hipError_t launch_sipg_cartesian(int degree, ApplyMode mode, SipgArgs<Number> co
     case ApplyMode::vmult: return dispatch_cartesian<Number, ApplyMode::vmult>(degree, a, s);
     case ApplyMode::vmult_add: return dispatch_cartesian<Number, ApplyMode::vmult_add>(degree, a, s);
     case ApplyMode::diagonal: return dispatch_cartesian<Number, ApplyMode::diagonal>(degree, a, s);
+    case ApplyMode::chebyshev:
+      return dispatch_cartesian<Number, ApplyMode::chebyshev>(degree, a, s);
   }
   return hipErrorInvalidValue;
 }

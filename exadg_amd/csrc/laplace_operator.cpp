@@ -78,13 +78,8 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
 }
 
 template<typename Number>
-void LaplaceOperator<Number>::apply(ApplyMode mode, Number * dst, Number const * src, int probe,
-                                    hipStream_t s) const
+SipgArgs<Number> LaplaceOperator<Number>::make_args(Number * dst, Number const * src, int probe) const
 {
-  if(mode != ApplyMode::diagonal && (dst == nullptr || src == nullptr))
-    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
-  if(mode != ApplyMode::diagonal && (void const *)dst == (void const *)src)
-    EXADG_THROW(EXADG_ERR_ARGUMENT, "dst and src must not alias");
   SipgArgs<Number> a;
   a.src         = src;
   a.dst         = dst;
@@ -97,10 +92,47 @@ void LaplaceOperator<Number>::apply(ApplyMode mode, Number * dst, Number const *
   a.probe       = probe;
   // one tile of y-rows per XCD
   a.tile_rows = std::max(1, (grid_->topology().nc[1] + 7) / 8);
+  return a;
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::launch(ApplyMode mode, SipgArgs<Number> const & a, hipStream_t s) const
+{
   hipError_t const e = grid_->affine() ? launch_sipg_cartesian<Number>(data_.degree, mode, a, s)
                                        : launch_sipg_general<Number>(data_.degree, mode, a, s);
   if(e != hipSuccess)
     EXADG_THROW(EXADG_ERR_HIP, std::string("SIPG kernel launch: ") + hipGetErrorString(e));
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::apply(ApplyMode mode, Number * dst, Number const * src, int probe,
+                                    hipStream_t s) const
+{
+  if(mode != ApplyMode::diagonal && (dst == nullptr || src == nullptr))
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  if(mode != ApplyMode::diagonal && (void const *)dst == (void const *)src)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "dst and src must not alias");
+  launch(mode, make_args(dst, src, probe), s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::chebyshev_step(Number * x_new, Number const * x, Number const * x_old,
+                                             Number const * b, Number const * dinv, Number f1,
+                                             Number f2, bool has_old, hipStream_t s) const
+{
+  if(x_new == nullptr || x == nullptr || b == nullptr || dinv == nullptr || (has_old && !x_old))
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  if((void const *)x_new == (void const *)x || (void const *)b == (void const *)x_new ||
+     (void const *)dinv == (void const *)x_new)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "x_new must not alias x, b or dinv");
+  SipgArgs<Number> a = make_args(x_new, x, 0);
+  a.x_old            = x_old;
+  a.rhs              = b;
+  a.dinv             = dinv;
+  a.f1               = f1;
+  a.f2               = f2;
+  a.has_old          = has_old ? 1 : 0;
+  launch(ApplyMode::chebyshev, a, s);
 }
 
 template<typename Number>

@@ -66,6 +66,12 @@ public:
   // 608-646): exact diagonal by unit-vector probing of the cell-local block
   void calculate_diagonal(Number * diag, hipStream_t s) const;
   void calculate_inverse_diagonal(Number * diag, hipStream_t s) const;
+  // x_new = (1 + f1) x - f1 x_old (has_old) + f2 dinv (b - A x), the operator application and
+  // the Chebyshev-Jacobi update in one kernel (PreconditionChebyshev's inner step,
+  // chebyshev_smoother.h:37-227); x_new may alias x_old, not x
+  void chebyshev_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
+                      Number const * dinv, Number f1, Number f2, bool has_old,
+                      hipStream_t s) const;
 
   void vmult(void * dst, void const * src, hipStream_t s) const override
   {
@@ -97,6 +103,8 @@ public:
 
 private:
   void apply(ApplyMode mode, Number * dst, Number const * src, int probe, hipStream_t s) const;
+  SipgArgs<Number> make_args(Number * dst, Number const * src, int probe) const;
+  void             launch(ApplyMode mode, SipgArgs<Number> const & a, hipStream_t s) const;
 
   std::shared_ptr<StructuredGrid const> grid_;
   LaplaceOperatorData                   data_;

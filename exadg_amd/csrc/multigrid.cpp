@@ -147,6 +147,13 @@ public:
   {
     op_->calculate_inverse_diagonal(d, s);
   }
+  bool chebyshev_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
+                      Number const * dinv, Number f1, Number f2, bool has_old,
+                      hipStream_t s) const override
+  {
+    op_->chebyshev_step(x_new, x, x_old, b, dinv, f1, f2, has_old, s);
+    return true;
+  }
 
 private:
   std::unique_ptr<Poisson::LaplaceOperator<Number>> op_;
