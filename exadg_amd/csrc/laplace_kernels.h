@@ -18,14 +18,21 @@ enum class ApplyMode : int
   chebyshev = 3
 };
 
-// Table layout (Number): general kernel: [S n*n | Dq n*n | d0 n | d1 n];
-// Cartesian kernel: per direction d = 0,1,2 a block [h_d M n*n | K/h_d n*n | d0/h_d n | d1/h_d n].
+// Table layout (Number, SipgArgs::tab): general kernel: [S n*n | Dq n*n | d0 n | d1 n];
+// Cartesian kernel: [M n*n | K n*n | d0 n | d1 n] on the unit interval plus h per direction.
 template<typename Number>
 struct SipgArgs
 {
+  // 1D tables [A n*n | B n*n | c n | e n]. The Cartesian kernel reads them from this by-value
+  // copy, i.e. as uniform scalar kernel-argument loads (unit-interval mass M, stiffness K,
+  // end derivatives d0, d1, scaled per direction by h or 1/h inside the kernel); the general
+  // kernel (S, Dq, d0, d1) indexes them at run time and reads the device copy `tables`.
+  static constexpr int kMaxTable = 2 * 9 * 9 + 2 * 9;
+  Number         tab[kMaxTable];
+  Number         h[6];  // Cartesian kernel: h_x, h_y, h_z, 1/h_x, 1/h_y, 1/h_z
+  Number const * tables;  // general kernel: the same tables in device memory
   Number const * src;
   Number *       dst;
-  Number const * tables;
   Number const * cell_metric;  // general kernel only
   Number const * face_data;    // general kernel only
   Number const * tau_face;

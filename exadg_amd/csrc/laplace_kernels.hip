@@ -535,11 +535,13 @@ struct LdsLayout
   static constexpr int PC = (N == 4) ? 80 : (N == 6) ? 260 : (N == 7) ? 369 : (N == 8) ? 576 : N * N * N;
 };
 
+// a = L_d u along one line: tab holds the unit-interval K, d0, d1; hinv = 1/h_d. The neighbour
+// derivative traces dwnb are unit-interval derivatives as well (same h_d on a uniform box).
 template<int N, typename Number>
-__device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, Number const (&u)[N],
-                                               Number (&o)[N], int const (&type)[2],
-                                               Number const (&tau)[2], Number const (&wnb)[2],
-                                               Number const (&dwnb)[2])
+__device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, Number hinv,
+                                               Number const (&u)[N], Number (&o)[N],
+                                               int const (&type)[2], Number const (&tau)[2],
+                                               Number const (&wnb)[2], Number const (&dwnb)[2])
 {
   Number const * K  = tab + N * N;
   Number const * d0 = tab + 2 * N * N;
@@ -556,9 +558,20 @@ __device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, 
   Number const jl = Number(0.5) * (u[0] - wl), jr = Number(0.5) * (u[N - 1] - wr);
 #pragma unroll
   for(int i = 0; i < N; ++i)
-    o[i] += d0[i] * jl - d1[i] * jr;
-  o[0] += Number(0.5) * (du0 + dwl) + Number(2) * tau[0] * jl;
-  o[N - 1] += Number(2) * tau[1] * jr - Number(0.5) * (du1 + dwr);
+    o[i] = hinv * (o[i] + d0[i] * jl - d1[i] * jr);
+  o[0] += hinv * Number(0.5) * (du0 + dwl) + Number(2) * tau[0] * jl;
+  o[N - 1] += Number(2) * tau[1] * jr - hinv * Number(0.5) * (du1 + dwr);
+}
+
+// o = h M v along one line (unit-interval mass M = tab[0 .. N*N))
+template<int N, typename Number>
+__device__ __forceinline__ void mass_apply(Number const * __restrict__ M, Number h,
+                                           Number const (&v)[N], Number (&o)[N])
+{
+  mat_apply<N>(M, v, o);
+#pragma unroll
+  for(int i = 0; i < N; ++i)
+    o[i] *= h;
 }
 
 // Coordinates of the cell at traversal position pos0 + slot; the divisions act on the
@@ -598,7 +611,7 @@ __device__ __forceinline__ void traversal_coords(BoxTopology const & t, int tile
 template<int N, typename Number, ApplyMode mode>
 __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
 {
-  constexpr int N2 = N * N, N3 = N * N * N, TB = 2 * N * N + 2 * N;
+  constexpr int N2 = N * N, N3 = N * N * N;
   constexpr int CPB = cells_per_block(N);
   using L           = LdsLayout<N>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -607,11 +620,11 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   int const     lt     = threadIdx.x - slot * N2;
   int const     pos0   = (int)xcd_logical_block(blockIdx.x, gridDim.x) * CPB;
   bool const    active = slot < CPB && pos0 + slot < (int)a.n_cells;
-  // two LDS buffers per cell: U (the cell's values; from the Y phase on it holds B, whose
-  // lines each lane writes exactly where it read its own U line) and A
-  Number *      U      = smem + (active ? slot : 0) * 2 * L::PC;
+  // three LDS buffers per cell: U (the cell's values), A and B; before the X phase A and B
+  // stage the two y-neighbour cells
+  Number *      U      = smem + (active ? slot : 0) * 3 * L::PC;
   Number *      Abuf   = U + L::PC;
-  Number *      Bbuf   = U;
+  Number *      Bbuf   = Abuf + L::PC;
   int const     p = lt % N, q = lt / N;
   // LDS line bases of this lane for the x, y and z sweeps
   int const     bx = L::PX * p + L::PY * q, by = p + L::PY * q, bz = p + L::PX * q;
@@ -657,7 +670,7 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
 #pragma unroll
       for(int d = 0; d < 3; ++d)
       {
-        Number const * tb = a.tables + d * TB;
+        Number const * tb = a.tab;
         int const      gbase = d == 0 ? N * (p + N * q) : d == 1 ? p + N2 * q : p + N * q;
         int const      gstr  = d == 0 ? 1 : d == 1 ? N : N2;
         Number         w[N];
@@ -673,83 +686,100 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
     }
     else
     {
-      // all global loads up front: own cell and the six neighbour lines of this lane
+      // All global loads up front. The own cell and the z- and y-neighbour cells are read in
+      // the same pattern (lane lt reads DoFs lt + m N^2: consecutive lanes, consecutive
+      // addresses); a lane then holds z-lines, so the z traces are lane-local and the
+      // y-neighbour cells go through LDS (A, B) to be read as y-lines. Only x-neighbours at
+      // the workgroup's ends are read as strided x-lines; the others come from U.
       Number const * own_p = a.src + (size_t)cell * N3 + lt;
-#pragma unroll
-      for(int m = 0; m < N; ++m)
-        own[m] = own_p[m * N2];
-      // x-neighbours that are the previous / next cell of this workgroup are read from its
-      // LDS after the barrier (x-lines are the worst-coalesced global accesses)
       in_block_l = slot > 0 && cc[0] > 0;
       in_block_r = slot + 1 < CPB && cc[0] + 1 < a.topo.nc[0] && pos0 + slot + 1 < (int)a.n_cells;
-      Number w[3][2][N];
+      Number zl[N], zr[N], yl[N], yr[N], xl[N], xr[N];
+      Number const * pz0 = a.src + (size_t)(nb[2][0] >= 0 ? nb[2][0] : cell) * N3 + lt;
+      Number const * pz1 = a.src + (size_t)(nb[2][1] >= 0 ? nb[2][1] : cell) * N3 + lt;
+      Number const * py0 = a.src + (size_t)(nb[1][0] >= 0 ? nb[1][0] : cell) * N3 + lt;
+      Number const * py1 = a.src + (size_t)(nb[1][1] >= 0 ? nb[1][1] : cell) * N3 + lt;
 #pragma unroll
-      for(int d = 0; d < 3; ++d)
+      for(int m = 0; m < N; ++m)
       {
-        int const gbase = d == 0 ? N * (p + N * q) : d == 1 ? p + N2 * q : p + N * q;
-        int const gstr  = d == 0 ? 1 : d == 1 ? N : N2;
+        own[m] = own_p[m * N2];
+        zl[m]  = pz0[m * N2];
+        zr[m]  = pz1[m * N2];
+        yl[m]  = py0[m * N2];
+        yr[m]  = py1[m * N2];
+      }
+      int const xbase = N * (p + N * q);  // this lane's x-line (y = p, z = q)
+      {
+        Number const * px0 = a.src + (size_t)(nb[0][0] >= 0 ? nb[0][0] : cell) * N3 + xbase;
+        Number const * px1 = a.src + (size_t)(nb[0][1] >= 0 ? nb[0][1] : cell) * N3 + xbase;
 #pragma unroll
-        for(int s = 0; s < 2; ++s)
+        for(int i = 0; i < N; ++i)
         {
-          bool const skip = d == 0 && (s == 0 ? in_block_l : in_block_r);
-          if(!skip)
-          {
-            Number const * src = a.src + (size_t)(nb[d][s] >= 0 ? nb[d][s] : cell) * N3 + gbase;
-#pragma unroll
-            for(int i = 0; i < N; ++i)
-              w[d][s][i] = src[i * gstr];
-          }
-          else
-          {
-#pragma unroll
-            for(int i = 0; i < N; ++i)
-              w[d][s][i] = Number(0);
-          }
+          xl[i] = in_block_l ? Number(0) : px0[i];
+          xr[i] = in_block_r ? Number(0) : px1[i];
         }
       }
 #pragma unroll
       for(int m = 0; m < N; ++m)
-        U[bz + L::PY * m] = own[m];
-#pragma unroll
-      for(int d = 0; d < 3; ++d)
       {
-        Number const * tb = a.tables + d * TB;
-        wnb[d][0]         = w[d][0][N - 1];
-        dwnb[d][0]        = dot_line<N>(tb + 2 * N2 + N, w[d][0]);
-        wnb[d][1]         = w[d][1][0];
-        dwnb[d][1]        = dot_line<N>(tb + 2 * N2, w[d][1]);
+        U[bz + L::PY * m]    = own[m];
+        Abuf[bz + L::PY * m] = yl[m];
+        Bbuf[bz + L::PY * m] = yr[m];
       }
+      // lane-local traces (x ones are replaced from LDS for in-block neighbours)
+      Number const * tzb = a.tab;
+      Number const * txb = a.tab;
+      wnb[2][0]          = zl[N - 1];
+      dwnb[2][0]         = dot_line<N>(tzb + 2 * N2 + N, zl);
+      wnb[2][1]          = zr[0];
+      dwnb[2][1]         = dot_line<N>(tzb + 2 * N2, zr);
+      wnb[0][0]          = xl[N - 1];
+      dwnb[0][0]         = dot_line<N>(txb + 2 * N2 + N, xl);
+      wnb[0][1]          = xr[0];
+      dwnb[0][1]         = dot_line<N>(txb + 2 * N2, xr);
     }
   }
   __syncthreads();
-  Number const * tx = a.tables;
-  Number const * ty = a.tables + TB;
-  Number const * tz = a.tables + 2 * TB;
+  Number const * tx = a.tab;  // the same unit-interval tables in every direction
+  Number const * ty = a.tab;
+  Number const * tz = a.tab;
+  Number const   hx = a.h[0], hy = a.h[1], hz = a.h[2];
+  Number const   hxi = a.h[3], hyi = a.h[4], hzi = a.h[5];
 
   if constexpr(mode != ApplyMode::diagonal)
-    if(active && (in_block_l || in_block_r))
+  {
+    if(active)
     {
       Number w[N];
+      // y traces from the staged neighbour cells, read as this lane's y-line (x = p, z = q)
+      load_line<N>(Abuf + by, L::PX, w);
+      wnb[1][0]  = w[N - 1];
+      dwnb[1][0] = dot_line<N>(ty + 2 * N2 + N, w);
+      load_line<N>(Bbuf + by, L::PX, w);
+      wnb[1][1]  = w[0];
+      dwnb[1][1] = dot_line<N>(ty + 2 * N2, w);
       if(in_block_l)
       {
-        load_line<N>(U - 2 * L::PC + bx, 1, w);
+        load_line<N>(U - 3 * L::PC + bx, 1, w);
         wnb[0][0]  = w[N - 1];
         dwnb[0][0] = dot_line<N>(tx + 2 * N2 + N, w);
       }
       if(in_block_r)
       {
-        load_line<N>(U + 2 * L::PC + bx, 1, w);
+        load_line<N>(U + 3 * L::PC + bx, 1, w);
         wnb[0][1]  = w[0];
         dwnb[0][1] = dot_line<N>(tx + 2 * N2, w);
       }
     }
+    __syncthreads();  // A and B are overwritten from the X phase on
+  }
 
   // X: a_x = L_x u  -> A
   if(active)
   {
     Number u[N], o[N];
     load_line<N>(U + bx, 1, u);
-    line_laplacian<N>(tx, u, o, type[0], tau[0], wnb[0], dwnb[0]);
+    line_laplacian<N>(tx, hxi, u, o, type[0], tau[0], wnb[0], dwnb[0]);
     store_line<N>(Abuf + bx, 1, o);
   }
   __syncthreads();
@@ -758,10 +788,10 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   {
     Number u[N], o[N], v[N], w[N];
     load_line<N>(U + by, L::PX, u);
-    line_laplacian<N>(ty, u, o, type[1], tau[1], wnb[1], dwnb[1]);
+    line_laplacian<N>(ty, hyi, u, o, type[1], tau[1], wnb[1], dwnb[1]);
     store_line<N>(Bbuf + by, L::PX, o);
     load_line<N>(Abuf + by, L::PX, v);
-    mat_apply<N>(ty, v, w);
+    mass_apply<N>(ty, hy, v, w);
     store_line<N>(Abuf + by, L::PX, w);
   }
   __syncthreads();
@@ -770,7 +800,7 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   {
     Number v[N], w[N], r[N];
     load_line<N>(Bbuf + bx, 1, v);
-    mat_apply<N>(tx, v, w);
+    mass_apply<N>(tx, hx, v, w);
     load_line<N>(Abuf + bx, 1, r);
 #pragma unroll
     for(int i = 0; i < N; ++i)
@@ -783,9 +813,9 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   if(active)
   {
     Number o[N], v[N];
-    line_laplacian<N>(tz, own, o, type[2], tau[2], wnb[2], dwnb[2]);
+    line_laplacian<N>(tz, hzi, own, o, type[2], tau[2], wnb[2], dwnb[2]);
     load_line<N>(Abuf + bz, L::PY, v);
-    mat_apply<N>(tz, v, outp);
+    mass_apply<N>(tz, hz, v, outp);
     store_line<N>(Bbuf + bz, L::PY, o);
   }
   __syncthreads();
@@ -794,7 +824,7 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   {
     Number v[N], w[N];
     load_line<N>(Bbuf + bx, 1, v);
-    mat_apply<N>(tx, v, w);
+    mass_apply<N>(tx, hx, v, w);
     store_line<N>(Bbuf + bx, 1, w);
   }
   __syncthreads();
@@ -803,7 +833,7 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   {
     Number v[N], w[N];
     load_line<N>(Bbuf + by, L::PX, v);
-    mat_apply<N>(ty, v, w);
+    mass_apply<N>(ty, hy, v, w);
     store_line<N>(Bbuf + by, L::PX, w);
   }
   __syncthreads();
@@ -871,7 +901,7 @@ hipError_t launch_cartesian_n(SipgArgs<Number> const & a, hipStream_t stream)
 {
   constexpr int CPB    = cells_per_block(N);
   int const     blocks = (int)((a.n_cells + CPB - 1) / CPB);
-  size_t const  lds    = sizeof(Number) * CPB * 2 * LdsLayout<N>::PC;
+  size_t const  lds    = sizeof(Number) * CPB * 3 * LdsLayout<N>::PC;
   hipLaunchKernelGGL((sipg_cartesian_kernel<N, Number, mode>), dim3(blocks), dim3(256), lds, stream,
                      a);
   return hipGetLastError();

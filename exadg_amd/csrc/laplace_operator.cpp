@@ -20,34 +20,30 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
   bool const curved = !grid_->affine();
   geo_              = compute_geometry(*grid_, shape_, data.IP_factor, curved);
 
-  std::vector<Number> tab;
+  // 1D tables travel by value in the kernel arguments (uniform scalar loads)
+  std::vector<double> tab;
+  auto append = [&tab](std::vector<double> const & v) { tab.insert(tab.end(), v.begin(), v.end()); };
   if(curved)
   {
-    for(double v : shape_.S)
-      tab.push_back((Number)v);
-    for(double v : shape_.Dq)
-      tab.push_back((Number)v);
-    for(double v : shape_.d0)
-      tab.push_back((Number)v);
-    for(double v : shape_.d1)
-      tab.push_back((Number)v);
+    append(shape_.S);
+    append(shape_.Dq);
   }
   else
   {
-    for(int d = 0; d < 3; ++d)
-    {
-      double const h = grid_->h(d);
-      for(double v : shape_.mass)
-        tab.push_back((Number)(h * v));
-      for(double v : shape_.stiff)
-        tab.push_back((Number)(v / h));
-      for(double v : shape_.d0)
-        tab.push_back((Number)(v / h));
-      for(double v : shape_.d1)
-        tab.push_back((Number)(v / h));
-    }
+    append(shape_.mass);
+    append(shape_.stiff);
   }
-  tables_.upload(tab);
+  append(shape_.d0);
+  append(shape_.d1);
+  if((int)tab.size() > SipgArgs<Number>::kMaxTable)
+    EXADG_THROW(EXADG_ERR_INTERNAL, "1D tables exceed the kernel-argument table");
+  tab_.assign(tab.begin(), tab.end());
+  tables_.upload(tab_);
+  for(int d = 0; d < 3; ++d)
+  {
+    h_[d]     = (Number)grid_->h(d);
+    h_[3 + d] = (Number)(1.0 / grid_->h(d));
+  }
   {
     // FP64 tables of the divergence right-hand side: S, Sd = Dq S, then weights and points
     std::vector<double> dt(shape_.S);
@@ -81,9 +77,11 @@ template<typename Number>
 SipgArgs<Number> LaplaceOperator<Number>::make_args(Number * dst, Number const * src, int probe) const
 {
   SipgArgs<Number> a;
+  std::copy(tab_.begin(), tab_.end(), a.tab);
+  std::copy(h_, h_ + 6, a.h);
+  a.tables      = tables_.data();
   a.src         = src;
   a.dst         = dst;
-  a.tables      = tables_.data();
   a.cell_metric = cell_metric_.data();
   a.face_data   = face_data_.data();
   a.tau_face    = tau_face_.data();

@@ -111,6 +111,8 @@ private:
   ShapeInfo1D                           shape_;
   GeometryData                          geo_;  // host copy (tau, and metric for curved)
   int                                   n3_;
+  std::vector<Number>                   tab_;  // 1D tables (kernel-argument copy)
+  Number                                h_[6] = {1, 1, 1, 1, 1, 1};
   DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
   DeviceBuffer<double>                  div_tables_;          // S, Sd, weights|points (FP64)
   mutable DeviceBuffer<double>          modes_;
