@@ -145,6 +145,7 @@ def lib():
                                             ctypes.POINTER(vp), vp],
             "exadg_preconditioner_vmult": [vp, vp, vp, vp],
             "exadg_preconditioner_update": [vp, vp],
+            "exadg_preconditioner_smooth": [vp, vp, vp, vp, vp],
             "exadg_preconditioner_destroy": [vp],
             "exadg_preconditioner_mg_info": [vp, ip, ctypes.POINTER(ctypes.c_double), ip, ip],
             "exadg_cg_solve": [vp, vp, vp, vp, ctypes.POINTER(SolverData), ctypes.POINTER(SolveInfo),
@@ -298,6 +299,10 @@ class Preconditioner:
 
     def update(self, stream=None):
         check(lib().exadg_preconditioner_update(self._h, _stream(stream)))
+
+    def smooth(self, op, b, x, stream=None):
+        """x <- x + S (b - A x), S the finest-level smoother (multigrid only)."""
+        check(lib().exadg_preconditioner_smooth(op.handle, self._h, _ptr(b), _ptr(x), _stream(stream)))
 
     def mg_info(self):
         nl = ctypes.c_int()

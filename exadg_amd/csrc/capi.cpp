@@ -275,6 +275,25 @@ int exadg_preconditioner_vmult(exadg_preconditioner * p, double * dst, const dou
   });
 }
 
+int exadg_preconditioner_smooth(exadg_laplace * op, exadg_preconditioner * p, const double * b,
+                                double * x, void * stream)
+{
+  return guarded([&] {
+    require(op && p && b && x, "null argument");
+    require(p->p != nullptr, "the preconditioner has no smoother");
+    auto &            A = fp64_operator(op);
+    hipStream_t const s = stream_of(stream);
+    int64_t const     n = A.m();
+    ExaDG::DeviceVector<double> r(n), c(n);
+    A.vmult(r.data(), x, s);
+    ExaDG::vec::axpby<double>(r.data(), 1.0, b, -1.0, n, s);  // r = b - A x
+    if(!p->p->smoother_vmult(c.data(), r.data(), s))
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "only the multigrid preconditioner has a smoother");
+    ExaDG::vec::axpby<double>(x, 1.0, c.data(), 1.0, n, s);  // x += S r
+    HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int exadg_preconditioner_update(exadg_preconditioner * p, void * stream)
 {
   return guarded([&] {

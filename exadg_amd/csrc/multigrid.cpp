@@ -682,6 +682,23 @@ void MultigridPreconditioner<Number>::vmult_eager(double * dst, double const * s
 }
 
 template<typename Number>
+bool MultigridPreconditioner<Number>::smoother_vmult(double * dst, double const * src, hipStream_t s)
+{
+  Level &       L = levels_[n_levels() - 1];
+  int64_t const n = L.op->n();
+  if constexpr(std::is_same<Number, double>::value)
+    vec::copy(L.defect.data(), src, n, s);
+  else
+    vec::convert<Number, double>(L.defect.data(), src, n, s);
+  L.smoother.vmult(L.sol, L.sol_alt, L.defect.data(), s);
+  if constexpr(std::is_same<Number, double>::value)
+    vec::copy(dst, L.sol, n, s);
+  else
+    vec::convert<double, Number>(dst, L.sol, n, s);
+  return true;
+}
+
+template<typename Number>
 void MultigridPreconditioner<Number>::vmult(double * dst, double const * src, hipStream_t s)
 {
   // The first call runs eagerly (kernel attributes get set outside any capture). Later calls

@@ -27,6 +27,11 @@ public:
   virtual ~PreconditionerBase() = default;
   virtual void vmult(double * dst, double const * src, hipStream_t s) = 0;
   virtual void update(hipStream_t s)                                  = 0;
+  // dst = S src with S the finest-level smoother (zero initial guess); false if none
+  virtual bool smoother_vmult(double * /*dst*/, double const * /*src*/, hipStream_t /*s*/)
+  {
+    return false;
+  }
 };
 
 // point Jacobi (jacobi_preconditioner.h:33-82)
@@ -168,6 +173,7 @@ public:
   }
   void vmult(double * dst, double const * src, hipStream_t s) override;
   void update(hipStream_t s) override;
+  bool smoother_vmult(double * dst, double const * src, hipStream_t s) override;
 
   int n_levels() const { return (int)levels_.size(); }
   // level l: 0 = coarsest

@@ -84,10 +84,12 @@ def ab_test(ex, torch, op, prec, net, P, Bv, s_begin, s_end, rel_tol=1e-10,
             t0 = time.perf_counter()
             if arm == "zero":
                 x.zero_()
-            elif arm == "extrapolation":
+            elif arm.startswith("extrapolation"):
                 ex.extrapolate(hist, x)
             else:
                 fallbacks += int(net.predict(hist, b, x))
+            if arm.endswith("+smooth"):
+                prec.smooth(op, b, x)
             info = ex.cg_solve(op, b, x, prec, rel_tol=rel_tol, rel_mode=1)
             torch.cuda.synchronize()
             res[arm].append((info["iterations"], time.perf_counter() - t0, info["r0"] / bn,
@@ -98,8 +100,9 @@ def ab_test(ex, torch, op, prec, net, P, Bv, s_begin, s_end, rel_tol=1e-10,
                   "total_time_s": float(sum(r[1] for r in v)),
                   "mean_initial_rel_residual": float(np.mean([r[2] for r in v])),
                   "max_final_rel_residual": float(max(r[3] for r in v))}
-    if "learned" in out:
-        out["learned"]["fallbacks"] = fallbacks
+    for k in out:
+        if k.startswith("learned"):
+            out[k]["fallbacks"] = fallbacks
     return out
 
 

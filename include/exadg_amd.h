@@ -161,6 +161,12 @@ int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_des
 /* PreconditionerBase::vmult (preconditioner_base.h:33-67): dst = P^{-1} src, FP64 vectors. */
 int exadg_preconditioner_vmult(exadg_preconditioner * p, double * dst, const double * src,
                                void * stream);
+/* x <- x + S (b - A x), S the finest-level Chebyshev-Jacobi smoother of a MULTIGRID
+ * preconditioner applied with a zero initial guess (PreconditionChebyshev::vmult on the
+ * residual, in the V-cycle's precision; residual and update in FP64). No counterpart in the
+ * reference: it polishes an initial guess (the learned one) before the CG solve. */
+int exadg_preconditioner_smooth(exadg_laplace * op, exadg_preconditioner * p, const double * b,
+                                double * x, void * stream);
 /* PreconditionerBase::update: recompute diagonals / eigenvalue estimates. */
 int exadg_preconditioner_update(exadg_preconditioner * p, void * stream);
 int exadg_preconditioner_destroy(exadg_preconditioner * p);

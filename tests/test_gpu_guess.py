@@ -158,3 +158,34 @@ def test_operator_loss_needs_operator(torch_cuda, product):
     P = torch.zeros((6, 27 * 8), dtype=torch.float64, device="cuda")
     with pytest.raises(product.ExaDGError):
         net.train(P, P.clone(), 0, 6, 1, loss=product.GUESS_LOSS_ENERGY)
+
+
+def test_smoother_polish(torch_cuda, product):
+    """exadg_preconditioner_smooth: x <- x + S(b - A x) removes most of a rough error, is a
+    no-op at the solution, and is refused by a preconditioner without a smoother."""
+    torch = torch_cuda
+    op = product.LaplaceOperator(product.mesh_desc(4), product.bc_desc(), 3)
+    prec = product.Preconditioner(op, product.PRECOND_MULTIGRID,
+                                  product.mg_desc(3, precision=product.FP64,
+                                                  coarse_solver=product.COARSE_CHEBYSHEV))
+    xt = torch.from_numpy(oracle.random_vector(op.n)).cuda()
+    b = torch.empty_like(xt)
+    op.vmult(b, xt)
+    noise = torch.from_numpy(oracle.random_vector(op.n, seed=3)).cuda() * 1e-3
+    x = xt + noise
+    r = torch.empty_like(b)
+
+    def res(v):
+        op.vmult(r, v)
+        return float(torch.linalg.norm(b - r))
+    r0 = res(x)
+    prec.smooth(op, b, x)
+    torch.cuda.synchronize()
+    assert res(x) < 0.2 * r0
+    y = xt.clone()
+    prec.smooth(op, b, y)
+    torch.cuda.synchronize()
+    assert float((y - xt).abs().max()) <= 1e-10 * float(xt.abs().max())
+    jac = product.Preconditioner(op, product.PRECOND_POINT_JACOBI)
+    with pytest.raises(product.ExaDGError):
+        jac.smooth(op, b, x)
