@@ -134,6 +134,56 @@ def run_guess(ex, torch, args):
     }
 
 
+def run_combined(ex, torch, args):
+    """Config 5: the pressure sequence of config 4 on the curved mesh of config 3 (k=5,
+    deformation 0.1, MappingQ(5)), periodic x/y, Neumann walls z. Train on the first steps,
+    then A/B on the rest: FP32 V-cycle with the learned / extrapolated guess against the plain
+    FP64 method (FP64 V-cycle, extrapolated or zero guess)."""
+    from exadg_amd import pressure_sequence as ps
+
+    def log(msg):
+        print(f"[combined] {msg}", file=sys.stderr, flush=True)
+
+    k, nc = 5, args.combined_cells
+    t0 = time.perf_counter()
+    op, prec32 = ps.setup_box(ex, nc, k, ex.FP32, deformation=0.1, mapping_degree=k)
+    _, prec64 = ps.setup_box(ex, nc, k, ex.FP64, op=op)
+    t_setup = time.perf_counter() - t0
+    modes = ex.fourier_modes(2026, 1, 8)
+    log(f"{op.n} DoF, setup {t_setup:.1f} s; generating {args.combined_steps} steps")
+    t0 = time.perf_counter()
+    P, Bv, its, _ = ps.generate(ex, torch, op, prec32, modes, args.combined_steps, log=log)
+    t_gen = time.perf_counter() - t0
+    net = ex.GuessNet(k, history=3, width=128, seed=1)
+    log("training")
+    losses, t_train = ps.train(ex, net, torch, op, P, Bv, args.combined_train, args.combined_epochs,
+                               lr=1e-3, loss=ex.GUESS_LOSS_RESIDUAL)
+    log(f"train loss {losses[0]:.4g} -> {losses[-1]:.4g}; A/B")
+    s0, s1 = args.combined_train, args.combined_steps
+    ab32 = ps.ab_test(ex, torch, op, prec32, net, P, Bv, s0, s1, arms=("extrapolation", "learned"))
+    ab64 = ps.ab_test(ex, torch, op, prec64, None, P, Bv, s0, s1, arms=("zero", "extrapolation"))
+    n_ab = s1 - s0
+    per_step = lambda r: round(r["total_time_s"] / n_ab * 1e3, 2)
+    return {
+        "workload": f"pressure sequence on the curved mesh: [0,1]^3 deformed by 0.1 prod sin(pi x), "
+                    f"MappingQ(5), periodic x/y, Neumann walls z, {nc}^3 cells, degree 5, {op.n} DoF, "
+                    f"dt 0.01, {len(modes)} modes |n| <= 8",
+        "steps": args.combined_steps, "train_steps": s0, "setup_s": round(t_setup, 1),
+        "generation": {"time_s": round(t_gen, 2), "mean_iterations": float(np.mean(its[3:]))},
+        "training": {"epochs": args.combined_epochs, "time_s": round(t_train, 2),
+                     "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]},
+        "ms_per_step": {"fp32_mg_learned": per_step(ab32["learned"]),
+                        "fp32_mg_extrapolation": per_step(ab32["extrapolation"]),
+                        "fp64_mg_extrapolation_plain": per_step(ab64["extrapolation"]),
+                        "fp64_mg_zero_guess": per_step(ab64["zero"])},
+        "fp32_mg": ab32, "fp64_mg": ab64,
+        "speedup_learned_fp32mg_vs_plain_fp64mg_extrapolated": round(
+            ab64["extrapolation"]["total_time_s"] / ab32["learned"]["total_time_s"], 3),
+        "speedup_learned_fp32mg_vs_fp64mg_zero_guess": round(
+            ab64["zero"]["total_time_s"] / ab32["learned"]["total_time_s"], 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +205,12 @@ def main():
     ap.add_argument("--guess-epochs", type=int, default=20)
     ap.add_argument("--guess-kmax", type=int, default=8)
     ap.add_argument("--guess-lr", type=float, default=1e-3)
+    ap.add_argument("--combined", action="store_true",
+                    help="also run the combined experiment (config 5): curved k=5 pressure sequence")
+    ap.add_argument("--combined-cells", type=int, default=50)
+    ap.add_argument("--combined-steps", type=int, default=200)
+    ap.add_argument("--combined-train", type=int, default=150)
+    ap.add_argument("--combined-epochs", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run plain device copies of the same vector (HBM counter calibration)")
     args = ap.parse_args()
@@ -246,6 +302,10 @@ def main():
         x = y = None
         torch.cuda.empty_cache()
         out["learned_guess"] = run_guess(exadg_amd, torch, args)
+    if args.combined:
+        x = y = None
+        torch.cuda.empty_cache()
+        out["combined"] = run_combined(exadg_amd, torch, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.degree, 32)
     print(json.dumps(out), flush=True)

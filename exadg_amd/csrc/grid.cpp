@@ -330,4 +330,77 @@ GeometryData compute_geometry(StructuredGrid const & grid, ShapeInfo1D const & s
   return g;
 }
 
+DivergenceGeometry compute_divergence_geometry(StructuredGrid const & grid, ShapeInfo1D const & shape)
+{
+  DivergenceGeometry  g;
+  int const           n = shape.n, n2 = n * n, n3 = n2 * n;
+  int64_t const       nc = grid.n_cells();
+  BoxTopology const & topo = grid.topology();
+  g.cell.resize((size_t)nc * 12 * n3);
+  g.face.resize((size_t)topo.n_faces * 6 * n2);
+  parallel_range(nc, [&](int64_t b, int64_t e) {
+    std::vector<double> x(3 * n3), J(9 * n3);
+    for(int64_t c = b; c < e; ++c)
+    {
+      grid.evaluate(c, shape.qp, shape.qp, shape.qp, x.data(), J.data());
+      double * out = &g.cell[(size_t)c * 12 * n3];
+      for(int qz = 0; qz < n; ++qz)
+        for(int qy = 0; qy < n; ++qy)
+          for(int qx = 0; qx < n; ++qx)
+          {
+            int const q = qx + n * (qy + n * qz);
+            double    Jq[9], Ji[9];
+            for(int k = 0; k < 9; ++k)
+              Jq[k] = J[k * n3 + q];
+            double const jxw = inverse3(Jq, Ji) * shape.qw[qx] * shape.qw[qy] * shape.qw[qz];
+            for(int d = 0; d < 3; ++d)
+              out[d * n3 + q] = x[d * n3 + q];
+            for(int k = 0; k < 9; ++k)
+              out[(3 + k) * n3 + q] = jxw * Ji[k];
+          }
+    }
+  });
+  std::vector<double> const one0 = {0.0}, one1 = {1.0};
+  for(int d = 0; d < 3; ++d)
+  {
+    int const     t1 = tangent1(d), t2 = tangent2(d);
+    int64_t const nf = (int64_t)topo.layers[d] * topo.nc[t1] * topo.nc[t2];
+    parallel_range(nf, [&](int64_t b, int64_t e) {
+      std::vector<double> xf(3 * n2), Jf(9 * n2);
+      for(int64_t fi = b; fi < e; ++fi)
+      {
+        int const     fl = (int)(fi % topo.layers[d]);
+        int64_t const r  = fi / topo.layers[d];
+        int           cc[3];
+        cc[t1]           = (int)(r % topo.nc[t1]);
+        cc[t2]           = (int)(r / topo.nc[t1]);
+        bool const has_m = topo.periodic[d] || fl > 0;
+        // evaluate from the cell below the face (xi_d = 1), or above it on the lower boundary
+        cc[d] = has_m ? ((fl == 0) ? topo.nc[d] - 1 : fl - 1) : fl;
+        std::vector<double> const * p[3] = {&shape.qp, &shape.qp, &shape.qp};
+        p[d]                             = has_m ? &one1 : &one0;
+        grid.evaluate(cell_id(topo, cc), *p[0], *p[1], *p[2], xf.data(), Jf.data());
+        double * out = &g.face[(size_t)(topo.face_base[d] + fi) * 6 * n2];
+        for(int q = 0; q < n2; ++q)
+        {
+          double Jq[9], Ji[9];
+          for(int k = 0; k < 9; ++k)
+            Jq[k] = Jf[k * n2 + q];
+          double const det = inverse3(Jq, Ji);
+          double       nv[3] = {Ji[d * 3 + 0], Ji[d * 3 + 1], Ji[d * 3 + 2]};
+          double const nrm = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+          int const    qa = q % n, qb = q / n;
+          double const jxw = det * nrm * shape.qw[qa] * shape.qw[qb];
+          for(int k = 0; k < 3; ++k)
+          {
+            out[k * n2 + q]       = xf[k * n2 + q];
+            out[(3 + k) * n2 + q] = jxw * nv[k] / nrm;
+          }
+        }
+      }
+    });
+  }
+  return g;
+}
+
 }  // namespace ExaDG

@@ -58,4 +58,15 @@ struct GeometryData
 GeometryData compute_geometry(StructuredGrid const & grid, ShapeInfo1D const & shape,
                               double ip_factor, bool store_metric);
 
+// Point data of the weak divergence (right-hand side of the pressure step) on any mesh, FP64:
+// per cell 12 x n^3 values: the position x (3) and C = JxW J^{-1} (9, C[a][k] = JxW dxi_a/dx_k)
+// at the Gauss points; per face 6 x n^2 values: x (3) and JxW n (3), n the unit normal in +xi_d
+// direction, at the face Gauss points (t1 fastest), faces numbered as in BoxTopology.
+struct DivergenceGeometry
+{
+  std::vector<double> cell;
+  std::vector<double> face;
+};
+DivergenceGeometry compute_divergence_geometry(StructuredGrid const & grid, ShapeInfo1D const & shape);
+
 }  // namespace ExaDG

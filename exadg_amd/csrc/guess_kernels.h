@@ -57,6 +57,13 @@ hipError_t launch_fill_u32(unsigned int * p, unsigned int v, int n, hipStream_t 
 // weak DG divergence (central flux, no-slip walls) of u(x) = Re sum_m d_m c_m exp(i 2 pi n_m.x);
 // mode_data (device): 8 doubles per mode n_x, n_y, n_z, d[3], c_re, c_im with |n_d| <= K <= 16;
 // S, Sd: n x n value/derivative tables at the Gauss points; w: n weights then n points
+// the same on a curved mesh from compute_divergence_geometry's point data (K <= 16; LDS
+// 8 (5 n^3 + 768 (K+1)) bytes)
+hipError_t launch_divergence_rhs_curved(double * b, double const * mode_data, int n_modes, int K,
+                                        BoxTopology const & topo, int n, double const * S,
+                                        double const * Sd, double const * w,
+                                        double const * cell_geo, double const * face_geo,
+                                        int64_t n_cells, hipStream_t s);
 hipError_t launch_divergence_rhs(double * b, double const * mode_data, int n_modes, int K,
                                  BoxTopology const & topo, double const * lower, double const * h,
                                  int n, double const * S, double const * Sd, double const * w,

@@ -470,6 +470,144 @@ __global__ void __launch_bounds__(128)
     b[c * n3 + e] = o[e];
 }
 
+// Curved meshes: the same weak divergence from precomputed point data (position and
+// C = JxW J^{-1} per Gauss point, position and JxW n per face point). The field is evaluated
+// at arbitrary points: per thread a table of exp(i 2 pi m x_d), m = 0..K, in LDS (negative m
+// by conjugation), two complex products per mode.
+__global__ void __launch_bounds__(128)
+  divergence_rhs_curved_kernel(double * b, double const * __restrict__ md, int n_modes, int K,
+                               BoxTopology topo, int n, double const * S, double const * Sd,
+                               double const * w, double const * __restrict__ cgeo,
+                               double const * __restrict__ fgeo)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int const     n3 = n * n * n, n2 = n * n, nk = K + 1;
+  double *      f   = reinterpret_cast<double *>(smem_raw);  // 3 * n^3
+  double *      o   = f + 3 * n3;
+  double *      tmp = o + n3;
+  double *      E   = tmp + n3 + threadIdx.x * 6 * nk;      // this thread's table
+  int64_t const c   = blockIdx.x;
+  int           cc[3];
+  cell_coords(topo, c, cc);
+
+  auto velocity = [&](double const * x, double * u) {
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+      double sn, cs;
+      sincos(2.0 * M_PI * x[d], &sn, &cs);
+      double * e = E + d * 2 * nk;
+      e[0]       = 1.0;
+      e[1]       = 0.0;
+      for(int m = 1; m <= K; ++m)
+      {
+        e[2 * m]     = e[2 * m - 2] * cs - e[2 * m - 1] * sn;
+        e[2 * m + 1] = e[2 * m - 2] * sn + e[2 * m - 1] * cs;
+      }
+    }
+    u[0] = u[1] = u[2] = 0;
+    for(int m = 0; m < n_modes; ++m)
+    {
+      double const * q = md + 8 * m;
+      int const      a0 = (int)q[0], a1 = (int)q[1], a2 = (int)q[2];
+      double const * ex = E + 2 * (a0 < 0 ? -a0 : a0);
+      double const * ey = E + 2 * nk + 2 * (a1 < 0 ? -a1 : a1);
+      double const * ez = E + 4 * nk + 2 * (a2 < 0 ? -a2 : a2);
+      double const   xr = ex[0], xi = a0 < 0 ? -ex[1] : ex[1];
+      double const   yr = ey[0], yi = a1 < 0 ? -ey[1] : ey[1];
+      double const   zr0 = ez[0], zi0 = a2 < 0 ? -ez[1] : ez[1];
+      double const   ar = xr * yr - xi * yi, ai = xr * yi + xi * yr;
+      double const   zr = ar * zr0 - ai * zi0, zi = ar * zi0 + ai * zr0;
+      double const   v  = q[6] * zr - q[7] * zi;
+      u[0] += v * q[3];
+      u[1] += v * q[4];
+      u[2] += v * q[5];
+    }
+  };
+
+  // volume: f_a(q) = sum_k C[a][k] u_k(x_q), then o = -sum_a (grad_a test)^T f_a
+  double const * cg = cgeo + c * 12 * n3;
+  for(int q = threadIdx.x; q < n3; q += blockDim.x)
+  {
+    double const x[3] = {cg[q], cg[n3 + q], cg[2 * n3 + q]};
+    double       u[3];
+    velocity(x, u);
+    for(int a = 0; a < 3; ++a)
+      f[a * n3 + q] = cg[(3 + 3 * a) * n3 + q] * u[0] + cg[(4 + 3 * a) * n3 + q] * u[1] +
+                      cg[(5 + 3 * a) * n3 + q] * u[2];
+  }
+  for(int e = threadIdx.x; e < n3; e += blockDim.x)
+    o[e] = 0;
+  __syncthreads();
+  for(int d = 0; d < 3; ++d)
+  {
+    double const * Az = d == 2 ? Sd : S;
+    double const * Ay = d == 1 ? Sd : S;
+    double const * Ax = d == 0 ? Sd : S;
+    for(int e = threadIdx.x; e < n3; e += blockDim.x)
+    {
+      int const a = e % n, bb = (e / n) % n, k = e / n2;
+      double    sum = 0;
+      for(int cq = 0; cq < n; ++cq)
+        sum += Az[cq * n + k] * f[d * n3 + a + n * (bb + n * cq)];
+      tmp[e] = sum;
+    }
+    __syncthreads();
+    for(int e = threadIdx.x; e < n3; e += blockDim.x)
+    {
+      int const a = e % n, j = (e / n) % n, k = e / n2;
+      double    sum = 0;
+      for(int bq = 0; bq < n; ++bq)
+        sum += Ay[bq * n + j] * tmp[a + n * (bq + n * k)];
+      f[d * n3 + e] = sum;
+    }
+    __syncthreads();
+    for(int e = threadIdx.x; e < n3; e += blockDim.x)
+    {
+      int const i = e % n, j = (e / n) % n, k = e / n2;
+      double    sum = 0;
+      for(int aq = 0; aq < n; ++aq)
+        sum += Ax[aq * n + i] * f[d * n3 + aq + n * (j + n * k)];
+      o[e] -= sum;
+    }
+    __syncthreads();
+  }
+  // faces: outward flux u.n_K with n_K = +n on the face above the cell (s = 1), -n below
+  for(int d = 0; d < 3; ++d)
+    for(int s = 0; s < 2; ++s)
+    {
+      if(neighbor_id(topo, cc, d, s) < 0)
+        continue;
+      double const * fg = fgeo + face_id(topo, cc, d, s) * 6 * n2;
+      double const   sg = s ? 1.0 : -1.0;
+      int const      t1 = tangent1(d), t2 = tangent2(d);
+      for(int q = threadIdx.x; q < n2; q += blockDim.x)
+      {
+        double const x[3] = {fg[q], fg[n2 + q], fg[2 * n2 + q]};
+        double       u[3];
+        velocity(x, u);
+        tmp[q] = sg * (u[0] * fg[3 * n2 + q] + u[1] * fg[4 * n2 + q] + u[2] * fg[5 * n2 + q]);
+      }
+      __syncthreads();
+      for(int e = threadIdx.x; e < n2; e += blockDim.x)
+      {
+        int const i = e % n, j = e / n;
+        double    sum = 0;
+        for(int qb = 0; qb < n; ++qb)
+          for(int qa = 0; qa < n; ++qa)
+            sum += S[qa * n + i] * S[qb * n + j] * tmp[qa + n * qb];
+        int idx[3];
+        idx[d]  = s ? n - 1 : 0;
+        idx[t1] = i;
+        idx[t2] = j;
+        o[idx[0] + n * (idx[1] + n * idx[2])] += sum;
+      }
+      __syncthreads();
+    }
+  for(int e = threadIdx.x; e < n3; e += blockDim.x)
+    b[c * n3 + e] = o[e];
+}
+
 __global__ void __launch_bounds__(256)
   scale_to_float_kernel(float * dst, double const * src, double coef, int64_t n)
 {
@@ -619,6 +757,29 @@ hipError_t launch_divergence_rhs(double * b, double const * mode_data, int n_mod
   size_t const lds = sizeof(double) * (5 * n * n * n + 6 * (2 * K + 1) * (n + 2));
   hipLaunchKernelGGL(divergence_rhs_kernel, dim3(n_cells), dim3(128), lds, s, b, mode_data, n_modes,
                      K, topo, lower[0], lower[1], lower[2], h[0], h[1], h[2], n, S, Sd, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_divergence_rhs_curved(double * b, double const * mode_data, int n_modes, int K,
+                                        BoxTopology const & topo, int n, double const * S,
+                                        double const * Sd, double const * w,
+                                        double const * cell_geo, double const * face_geo,
+                                        int64_t n_cells, hipStream_t s)
+{
+  if(n < 1 || n > 9 || K < 0 || K > 16 || n_cells < 1)
+    return hipErrorInvalidValue;
+  size_t const lds = sizeof(double) * (5 * n * n * n + 128 * 6 * (K + 1));
+  static size_t attr = 0;
+  if(lds > 65536 && lds > attr)
+  {
+    hipError_t e = hipFuncSetAttribute((void const *)divergence_rhs_curved_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if(e != hipSuccess)
+      return e;
+    attr = lds;
+  }
+  hipLaunchKernelGGL(divergence_rhs_curved_kernel, dim3(n_cells), dim3(128), lds, s, b, mode_data,
+                     n_modes, K, topo, n, S, Sd, w, cell_geo, face_geo);
   return hipGetLastError();
 }
 

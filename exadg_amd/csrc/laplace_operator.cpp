@@ -190,8 +190,6 @@ void LaplaceOperator<Number>::rhs_divergence_fourier(double const * modes_host, 
 {
   if(b == nullptr || modes_host == nullptr || n_modes < 1)
     EXADG_THROW(EXADG_ERR_ARGUMENT, "invalid Fourier field");
-  if(!grid_->affine())
-    EXADG_THROW(EXADG_ERR_ARGUMENT, "the divergence right-hand side needs an affine mesh");
   // per-call mode data: n, d, and c = A exp(i (phi + omega t)), so that the kernel evaluates
   // u = sum A d cos(2 pi n.x + phi + omega t) = Re sum d c exp(i 2 pi n.x)
   std::vector<double> mh((size_t)8 * n_modes);
@@ -216,10 +214,24 @@ void LaplaceOperator<Number>::rhs_divergence_fourier(double const * modes_host, 
   HIP_CHECK(hipMemcpyAsync(modes_.data(), mh.data(), mh.size() * sizeof(double), hipMemcpyHostToDevice, s));
   int const    n    = shape_.n;
   double const h[3] = {grid_->h(0), grid_->h(1), grid_->h(2)};
-  hipError_t   e    = launch_divergence_rhs(b, modes_.data(), n_modes, K, grid_->topology(),
-                                            grid_->mesh().lower, h, n, div_tables_.data(),
-                                            div_tables_.data() + n * n,
-                                            div_tables_.data() + 2 * n * n, grid_->n_cells(), s);
+  hipError_t   e    = hipSuccess;
+  if(grid_->affine())
+    e = launch_divergence_rhs(b, modes_.data(), n_modes, K, grid_->topology(), grid_->mesh().lower,
+                              h, n, div_tables_.data(), div_tables_.data() + n * n,
+                              div_tables_.data() + 2 * n * n, grid_->n_cells(), s);
+  else
+  {
+    if(div_cell_geo_.size() == 0)  // point data of the curved mesh, computed on first use
+    {
+      DivergenceGeometry const g = compute_divergence_geometry(*grid_, shape_);
+      div_cell_geo_.upload(g.cell);
+      div_face_geo_.upload(g.face);
+    }
+    e = launch_divergence_rhs_curved(b, modes_.data(), n_modes, K, grid_->topology(), n,
+                                     div_tables_.data(), div_tables_.data() + n * n,
+                                     div_tables_.data() + 2 * n * n, div_cell_geo_.data(),
+                                     div_face_geo_.data(), grid_->n_cells(), s);
+  }
   if(e != hipSuccess)
     EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
   if(is_singular())

@@ -38,7 +38,7 @@ public:
   virtual void    calculate_inverse_diagonal(void * diag, hipStream_t s) const = 0;
   virtual void    rhs_sine(double omega, double * b, hipStream_t s) const      = 0;
   // weak DG divergence (central flux, no-slip walls) of a Fourier velocity field; mean
-  // removed when the operator is singular (affine meshes)
+  // removed when the operator is singular
   virtual void rhs_divergence_fourier(double const * modes_host, int n_modes, double t, double * b,
                                       hipStream_t s) const = 0;
   virtual bool is_singular() const                                             = 0;
@@ -115,6 +115,7 @@ private:
   Number                                h_[6] = {1, 1, 1, 1, 1, 1};
   DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
   DeviceBuffer<double>                  div_tables_;          // S, Sd, weights|points (FP64)
+  mutable DeviceBuffer<double>          div_cell_geo_, div_face_geo_;  // curved meshes
   mutable DeviceBuffer<double>          modes_;
   mutable std::unique_ptr<Reducer>      reducer_;
 };

@@ -17,12 +17,20 @@ import time
 import numpy as np
 
 
-def setup_box(ex, n_cells, degree, precision_mg, coarse_solver=None):
-    """Periodic-x/y, Neumann-z box on [0,1]^3 with CG + hybrid p-MG; the coarse Q1 problem is
-    solved by Chebyshev (MGCoarseChebyshev) unless stated otherwise."""
+def setup_box(ex, n_cells, degree, precision_mg, coarse_solver=None, deformation=0.0,
+              mapping_degree=1, op=None):
+    """Periodic-x/y, Neumann-z box on [0,1]^3 (optionally deformed by the DeformedCubeManifold
+    map, MappingQ(mapping_degree)) with CG + hybrid p-MG; the coarse Q1 problem is solved by
+    Chebyshev (MGCoarseChebyshev) unless stated otherwise. An existing operator may be passed
+    to build only another preconditioner."""
     if coarse_solver is None:
         coarse_solver = ex.COARSE_CHEBYSHEV
-    mesh = ex.mesh_desc(n_cells, periodic=(1, 1, 0))
+    if op is not None:
+        return op, ex.Preconditioner(op, ex.PRECOND_MULTIGRID,
+                                     ex.mg_desc(degree, precision=precision_mg, singular=True,
+                                                coarse_solver=coarse_solver))
+    mesh = ex.mesh_desc(n_cells, periodic=(1, 1, 0), deformation=deformation,
+                        mapping_degree=mapping_degree)
     bc = ex.bc_desc((ex.NEUMANN,) * 6)
     op = ex.LaplaceOperator(mesh, bc, degree)
     prec = ex.Preconditioner(op, ex.PRECOND_MULTIGRID,

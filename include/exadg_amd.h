@@ -216,8 +216,9 @@ int exadg_fourier_modes(uint64_t seed, int k_min, int k_max, int max_modes, doub
 
 /* Right-hand side of the pressure step: the weak DG divergence with central flux and
  * no-slip walls (divergence_operator.h:63-68, divergence_operator.cpp:168-298) of the Fourier
- * field at time t, mean removed when the operator is singular (no Dirichlet side). Affine
- * meshes; b is FP64 (device). */
+ * field at time t, mean removed when the operator is singular (no Dirichlet side). Affine and
+ * curved meshes (the first call on a curved mesh computes its point data); b is FP64
+ * (device). */
 int exadg_laplace_rhs_divergence_fourier(exadg_laplace * op, const double * modes, int n_modes,
                                          double t, double * b, void * stream);
 

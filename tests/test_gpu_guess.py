@@ -24,22 +24,26 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("k,nc,periodic,bc", [
-    (3, 4, (1, 1, 0), (1,) * 6),
-    (4, 3, (1, 1, 0), (1,) * 6),
-    (2, (3, 4, 2), (0, 1, 0), (0, 1, 1, 1, 0, 1)),
-    (5, 2, (1, 1, 1), (0,) * 6),
+@pytest.mark.parametrize("k,nc,periodic,bc,dfm,mdeg", [
+    (3, 4, (1, 1, 0), (1,) * 6, 0.0, 1),
+    (4, 3, (1, 1, 0), (1,) * 6, 0.0, 1),
+    (2, (3, 4, 2), (0, 1, 0), (0, 1, 1, 1, 0, 1), 0.0, 1),
+    (5, 2, (1, 1, 1), (0,) * 6, 0.0, 1),
+    (3, 3, (1, 1, 0), (1,) * 6, 0.1, 3),
+    (5, 2, (0, 0, 0), (0, 1, 0, 0, 1, 0), 0.08, 5),
 ])
-def test_divergence_rhs_vs_oracle(torch_cuda, product, k, nc, periodic, bc):
+def test_divergence_rhs_vs_oracle(torch_cuda, product, k, nc, periodic, bc, dfm, mdeg):
     torch = torch_cuda
     modes = product.fourier_modes(21, 1, 3)
-    mesh = product.mesh_desc(nc, periodic=periodic)
+    mesh = product.mesh_desc(nc, periodic=periodic, deformation=dfm, mapping_degree=mdeg)
     op = product.LaplaceOperator(mesh, product.bc_desc(bc), k)
+    assert op.is_affine == (dfm == 0.0)
     b = torch.empty(op.n, dtype=torch.float64, device="cuda")
     op.rhs_divergence_fourier(modes, 0.37, b)
     torch.cuda.synchronize()
     ncs = nc if isinstance(nc, tuple) else (nc,) * 3
-    ref = oracle.LaplaceOperator(oracle.make_mesh(ncs, periodic=periodic, boundary_type=bc), k)
+    ref = oracle.LaplaceOperator(oracle.make_mesh(ncs, periodic=periodic, boundary_type=bc,
+                                                  deformation=dfm, mapping_degree=mdeg), k)
     bo = ref.rhs_divergence_fourier(modes, 0.37)
     assert _rel(b.cpu().numpy(), bo) < 1e-12
 
