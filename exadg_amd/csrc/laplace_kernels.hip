@@ -138,7 +138,9 @@ constexpr int general_work_size()
 template<int N, typename Number, ApplyMode mode>
 __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> const & a,
                                                        int64_t cell, int const * cc, bool active,
-                                                       int lt, Number * A, Number * W)
+                                                       int lt, Number * A, Number * W,
+                                                       Number const (&nbw)[2],
+                                                       Number const (&nbdw)[2])
 {
   constexpr int  N2 = N * N, N3 = N * N * N;
   Number const * S  = a.tables;
@@ -172,19 +174,16 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
     E[5 * N2 + lt] = dot_line<N>(d1, u);
     if constexpr(mode != ApplyMode::diagonal)
     {
+      // neighbour traces, read at kernel start (their loads overlap the cell phases)
       if(type[0] == kInterior)
       {
-        Number w[N];
-        load_line<N>(a.src + nb[0] * N3 + base, stride, w);
-        E[2 * N2 + lt] = w[N - 1];
-        E[3 * N2 + lt] = dot_line<N>(d1, w);
+        E[2 * N2 + lt] = nbw[0];
+        E[3 * N2 + lt] = nbdw[0];
       }
       if(type[1] == kInterior)
       {
-        Number w[N];
-        load_line<N>(a.src + nb[1] * N3 + base, stride, w);
-        E[6 * N2 + lt] = w[0];
-        E[7 * N2 + lt] = dot_line<N>(d0, w);
+        E[6 * N2 + lt] = nbw[1];
+        E[7 * N2 + lt] = nbdw[1];
       }
     }
     else
@@ -365,6 +364,34 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
   if(active)
     cell_coords(a.topo, cell, cc);
 
+  // neighbour traces of this lane's line in every direction: value and derivative at the
+  // shared face, read up front so that the loads overlap the cell phases
+  Number nbw[3][2] = {}, nbdw[3][2] = {};
+  if constexpr(mode != ApplyMode::diagonal)
+    if(active)
+    {
+      Number const * d0 = a.tables + 2 * N2;
+      Number const * d1 = d0 + N;
+#pragma unroll
+      for(int d = 0; d < 3; ++d)
+      {
+        int base, stride;
+        line_geometry<N>(d, p, q, base, stride);
+#pragma unroll
+        for(int sd = 0; sd < 2; ++sd)
+        {
+          int64_t const nbc = neighbor_id(a.topo, cc, d, sd);
+          if(nbc >= 0)
+          {
+            Number w[N];
+            load_line<N>(a.src + nbc * N3 + base, stride, w);
+            nbw[d][sd]  = sd ? w[0] : w[N - 1];
+            nbdw[d][sd] = dot_line<N>(sd ? d0 : d1, w);
+          }
+        }
+      }
+    }
+
   // load the cell values
   if(active)
   {
@@ -483,8 +510,9 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
   }
   __syncthreads();
 
+#pragma unroll
   for(int d = 0; d < 3; ++d)
-    general_face_direction<N, Number, mode>(d, a, cell, cc, active, lt, A, W);
+    general_face_direction<N, Number, mode>(d, a, cell, cc, active, lt, A, W, nbw[d], nbdw[d]);
 
   if(active)
   {
