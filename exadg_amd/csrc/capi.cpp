@@ -71,6 +71,7 @@ struct exadg_preconditioner
 {
   int                                        kind = 0;
   std::unique_ptr<ExaDG::PreconditionerBase> p;
+  ExaDG::DeviceVector<double>                work_r, work_c;  // exadg_preconditioner_smooth
 };
 
 namespace
@@ -284,12 +285,15 @@ int exadg_preconditioner_smooth(exadg_laplace * op, exadg_preconditioner * p, co
     auto &            A = fp64_operator(op);
     hipStream_t const s = stream_of(stream);
     int64_t const     n = A.m();
-    ExaDG::DeviceVector<double> r(n), c(n);
-    A.vmult(r.data(), x, s);
-    ExaDG::vec::axpby<double>(r.data(), 1.0, b, -1.0, n, s);  // r = b - A x
-    if(!p->p->smoother_vmult(c.data(), r.data(), s))
+    p->work_r.reinit(n);
+    p->work_c.reinit(n);
+    double * r = p->work_r.data();
+    double * c = p->work_c.data();
+    A.vmult(r, x, s);
+    ExaDG::vec::axpby<double>(r, 1.0, b, -1.0, n, s);  // r = b - A x
+    if(!p->p->smoother_vmult(c, r, s))
       EXADG_THROW(EXADG_ERR_ARGUMENT, "only the multigrid preconditioner has a smoother");
-    ExaDG::vec::axpby<double>(x, 1.0, c.data(), 1.0, n, s);  // x += S r
+    ExaDG::vec::axpby<double>(x, 1.0, c, 1.0, n, s);  // x += S r
     HIP_CHECK(hipStreamSynchronize(s));
   });
 }
