@@ -150,6 +150,7 @@ def lib():
             "exadg_preconditioner_mg_info": [vp, ip, ctypes.POINTER(ctypes.c_double), ip, ip],
             "exadg_cg_solve": [vp, vp, vp, vp, ctypes.POINTER(SolverData), ctypes.POINTER(SolveInfo),
                                vp],
+            "exadg_laplace_error_sine": [vp, ctypes.c_double, vp, dp, vp],
             "exadg_fourier_modes": [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ip],
             "exadg_laplace_rhs_divergence_fourier": [vp, dp, ctypes.c_int, ctypes.c_double, vp, vp],
             "exadg_vector_subtract_mean": [vp, ctypes.c_int64, vp],
@@ -255,6 +256,15 @@ class LaplaceOperator:
         import torch
         assert b.is_cuda and b.dtype == torch.float64 and b.numel() == self.n
         check(lib().exadg_laplace_rhs_sine(self._h, omega, _ptr(b), _stream(stream)))
+
+    def error_sine(self, omega, u, stream=None):
+        """Relative L2 error against prod sin(omega x_d), QGauss(k+3) (FP64 operator)."""
+        import torch
+        assert u.is_cuda and u.dtype == torch.float64 and u.numel() == self.n
+        e = ctypes.c_double()
+        check(lib().exadg_laplace_error_sine(self._h, omega, _ptr(u), ctypes.byref(e),
+                                             _stream(stream)))
+        return e.value
 
     def rhs_divergence_fourier(self, modes, t, b, stream=None):
         """Weak DG divergence (central flux, no-slip walls) of the Fourier field `modes`

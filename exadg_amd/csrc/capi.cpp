@@ -325,6 +325,15 @@ int exadg_preconditioner_mg_info(const exadg_preconditioner * p, int * n_levels,
   });
 }
 
+int exadg_laplace_error_sine(exadg_laplace * op, double omega, const double * u, double * rel_error,
+                             void * stream)
+{
+  return guarded([&] {
+    require(op && u && rel_error, "null argument");
+    *rel_error = op->op->error_sine(omega, u, stream_of(stream));
+  });
+}
+
 int exadg_fourier_modes(uint64_t seed, int k_min, int k_max, int max_modes, double * modes,
                         int * n_modes)
 {

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace ExaDG
 {
@@ -56,6 +57,21 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
     dt.insert(dt.end(), shape_.qw.begin(), shape_.qw.end());
     dt.insert(dt.end(), shape_.qp.begin(), shape_.qp.end());
     div_tables_.upload(dt);
+  }
+  {
+    // error norm with QGauss(k+3): Se[q][i] = phi_i(e_q), then weights, then points
+    int const           ne = n + 2;
+    std::vector<double> ep, ew, Se((size_t)ne * n), val(n), der(n);
+    gauss_points(ne, ep, ew);
+    for(int q = 0; q < ne; ++q)
+    {
+      lagrange_basis(shape_.nodes, ep[q], val.data(), der.data());
+      for(int i = 0; i < n; ++i)
+        Se[q * n + i] = val[i];
+    }
+    Se.insert(Se.end(), ew.begin(), ew.end());
+    Se.insert(Se.end(), ep.begin(), ep.end());
+    err_tables_.upload(Se);
   }
   auto convert = [](std::vector<double> const & v) {
     return std::vector<Number>(v.begin(), v.end());
@@ -241,6 +257,68 @@ void LaplaceOperator<Number>::rhs_divergence_fourier(double const * modes_host, 
     vec::subtract_mean<double>(*reducer_, b, m(), s);
   }
   HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template<typename Number>
+double LaplaceOperator<Number>::error_sine(double omega, double const * u, hipStream_t s) const
+{
+  if(!std::is_same<Number, double>::value)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "the error norm needs the FP64 operator");
+  if(u == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  int const     n = shape_.n, ne = n + 2, ne3 = ne * ne * ne;
+  int64_t const nc = grid_->n_cells();
+  double const * Se = err_tables_.data();
+  double const * w  = Se + ne * n;
+  if(!grid_->affine() && err_geo_.size() == 0)
+  {
+    // positions and JxW at the error points of every cell, computed once
+    std::vector<double> hw(2 * ne);
+    HIP_CHECK(hipMemcpy(hw.data(), w, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost));
+    std::vector<double> ew(hw.begin(), hw.begin() + ne), epts(hw.begin() + ne, hw.end());
+    std::vector<double> geo((size_t)nc * 4 * ne3);
+    parallel_range(nc, [&](int64_t b, int64_t e) {
+      std::vector<double> x(3 * ne3), J(9 * ne3);
+      for(int64_t c = b; c < e; ++c)
+      {
+        grid_->evaluate(c, epts, epts, epts, x.data(), J.data());
+        double * g = &geo[(size_t)c * 4 * ne3];
+        for(int qz = 0; qz < ne; ++qz)
+          for(int qy = 0; qy < ne; ++qy)
+            for(int qx = 0; qx < ne; ++qx)
+            {
+              int const q = qx + ne * (qy + ne * qz);
+              double    a[9];
+              for(int k = 0; k < 9; ++k)
+                a[k] = J[k * ne3 + q];
+              double const det = a[0] * (a[4] * a[8] - a[5] * a[7]) -
+                                 a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                                 a[2] * (a[3] * a[7] - a[4] * a[6]);
+              for(int d = 0; d < 3; ++d)
+                g[d * ne3 + q] = x[d * ne3 + q];
+              g[3 * ne3 + q] = std::abs(det) * ew[qx] * ew[qy] * ew[qz];
+            }
+      }
+    });
+    err_geo_.upload(geo);
+  }
+  if(err_partial_.size() < (size_t)(2 * nc))
+    err_partial_.resize((size_t)(2 * nc));
+  if(err_scratch_.size() < 2 * 1024 + 2)
+    err_scratch_.resize(2 * 1024 + 2);
+  double const h[3] = {grid_->h(0), grid_->h(1), grid_->h(2)};
+  hipError_t   e    = launch_error_sine(err_partial_.data(), u, omega, n, ne, Se, w,
+                                        grid_->topology(), grid_->mesh().lower, h,
+                                        grid_->affine() ? nullptr : err_geo_.data(), nc, s);
+  if(e == hipSuccess)
+    e = launch_sum_pairs(err_partial_.data(), nc, err_scratch_.data(),
+                         err_scratch_.data() + 2 * 1024, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  double sums[2];
+  HIP_CHECK(hipMemcpyAsync(sums, err_scratch_.data() + 2 * 1024, sizeof(sums), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return sums[1] > 0 ? std::sqrt(sums[0] / sums[1]) : std::sqrt(sums[0]);
 }
 
 std::unique_ptr<LaplaceOperatorBase> create_laplace_operator(exadg_mesh_desc const & mesh,

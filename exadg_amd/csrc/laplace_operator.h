@@ -7,6 +7,7 @@
 
 #include "common.h"
 #include "grid.h"
+#include "error_kernels.h"
 #include "guess_kernels.h"
 #include "laplace_kernels.h"
 #include "shape_info.h"
@@ -42,6 +43,9 @@ public:
   virtual void rhs_divergence_fourier(double const * modes_host, int n_modes, double t, double * b,
                                       hipStream_t s) const = 0;
   virtual bool is_singular() const                                             = 0;
+  // relative L2 error of u (FP64, device) against prod_d sin(omega x_d), QGauss(k+3)
+  // (calculate_error, error_calculation.cpp:38-103); FP64 operators only
+  virtual double error_sine(double omega, double const * u, hipStream_t s) const = 0;
   virtual std::shared_ptr<StructuredGrid const> grid_ptr() const               = 0;
   virtual double                                ip_factor() const              = 0;
 };
@@ -93,7 +97,8 @@ public:
   void rhs_sine(double omega, double * b, hipStream_t s) const override;
   void rhs_divergence_fourier(double const * modes_host, int n_modes, double t, double * b,
                               hipStream_t s) const override;
-  bool is_singular() const override;
+  bool   is_singular() const override;
+  double error_sine(double omega, double const * u, hipStream_t s) const override;
 
   StructuredGrid const & grid() const { return *grid_; }
   std::shared_ptr<StructuredGrid const> grid_ptr() const override { return grid_; }
@@ -116,6 +121,9 @@ private:
   DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
   DeviceBuffer<double>                  div_tables_;          // S, Sd, weights|points (FP64)
   mutable DeviceBuffer<double>          div_cell_geo_, div_face_geo_;  // curved meshes
+  // error norm: Se (ne x n) | weights | points of QGauss(k+3); curved meshes: x and JxW per point
+  DeviceBuffer<double>                  err_tables_;
+  mutable DeviceBuffer<double>          err_geo_, err_partial_, err_scratch_;
   mutable DeviceBuffer<double>          modes_;
   mutable std::unique_ptr<Reducer>      reducer_;
 };

@@ -204,6 +204,13 @@ typedef struct exadg_solve_info
 int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
                    const exadg_solver_data * data, exadg_solve_info * info, void * stream);
 
+/* Relative L2 error of u (FP64, device) against u* = prod_d sin(omega x_d) with QGauss(k+3)
+ * on the mapped cells: the number the reference's sine application prints
+ * (ErrorCalculator::calculate_error, postprocessor/error_calculation.cpp:38-103). FP64
+ * operators; the first call on a curved mesh computes its point data. */
+int exadg_laplace_error_sine(exadg_laplace * op, double omega, const double * u, double * rel_error,
+                             void * stream);
+
 /* ---------------------------------------------------------------- pressure sequences */
 
 /* Seeded solenoidal Fourier velocity field u(x,t) = sum_m A_m d_m cos(2 pi n_m.x + phi_m + omega_m t)

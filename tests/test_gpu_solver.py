@@ -165,8 +165,11 @@ def test_pcg_multigrid_golden_error(mesh, precision, torch_cuda, product):
     bh = b.cpu().numpy()
     assert np.linalg.norm(bh - ref.vmult(xs)) <= 2e-10 * np.linalg.norm(bh)
     err = ref.error_sine(3 * np.pi, xs)
-    print(f"{mesh} {precision}: {info['iterations']} its, err {err:.5e}", flush=True)
+    err_gpu = op.error_sine(3 * np.pi, x)
+    print(f"{mesh} {precision}: {info['iterations']} its, err {err:.5e} (GPU {err_gpu:.5e})", flush=True)
     assert f"{err:.5e}" == f"{GOLD[mesh][str(k)]:.5e}"
+    assert f"{err_gpu:.5e}" == f"{GOLD[mesh][str(k)]:.5e}"
+    assert abs(err_gpu - err) <= 1e-10 * err
     assert info["iterations"] <= 25
 
 

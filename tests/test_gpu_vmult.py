@@ -140,3 +140,18 @@ def test_vmult_full_size_affine_k4(torch_cuda, product):
     a = float(torch.dot(zs, ys))
     b = float(torch.dot(xs, ws))
     assert abs(a - b) <= 1e-11 * (abs(a) + abs(b))
+
+
+@pytest.mark.parametrize("k,curved", [(2, False), (4, False), (3, True), (6, True)])
+def test_error_sine_vs_oracle(torch_cuda, product, k, curved):
+    """exadg_laplace_error_sine equals the oracle's QGauss(k+3) error norm (calculate_error)."""
+    torch = torch_cuda
+    dfm, md = (0.12, 3) if curved else (0.0, 1)
+    mp = product.mesh_desc((3, 4, 2), -1.0, 1.0, deformation=dfm, frequency=2, mapping_degree=md)
+    mo = oracle.make_mesh((3, 4, 2), -1.0, 1.0, deformation=dfm, frequency=2, mapping_degree=md)
+    op = product.LaplaceOperator(mp, product.bc_desc(), k)
+    ref = oracle.LaplaceOperator(mo, k)
+    u = oracle.random_vector(op.n) * 0.1 + 1.0
+    e_gpu = op.error_sine(1.3 * np.pi, torch.from_numpy(u).cuda())
+    e_ref = ref.error_sine(1.3 * np.pi, u)
+    assert abs(e_gpu - e_ref) <= 1e-12 * e_ref
