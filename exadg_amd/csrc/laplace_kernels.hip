@@ -550,8 +550,10 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
 //   left end : o_i += 1/2 d0_i [u], o_0     +=  1/2 (du + dw) + tau [u],  [u] = u_0 - w
 // For boundary faces (w, dw) come from the mirror principle (weak_boundary_conditions.h):
 // Dirichlet w = -u, dw = du; Neumann w = u, dw = -du -- one branch-free code path.
-// Evaluated as out = M_z (M_y a_x + M_x a_y) + M_y M_x a_z with a_d = L_d(u, traces).
-// The neighbour lines are read once, at kernel start, and reduced to their traces.
+// Evaluated as out = M_z (M_y a_x + M_x a_y) + M_y M_x a_z with a_d = L_d(u, traces) in four
+// LDS phases (five workgroup barriers): the three a_d from x-, y- and z-lines; M_x a_y, M_x a_z
+// (x-lines) and M_y a_x (y-lines); M_y M_x a_z (y-lines) and M_z (...) (z-lines); the sum along
+// the z-lines. The neighbour lines are read once, at kernel start, and reduced to their traces.
 
 // Padded LDS strides (x, y, z, cell) per n, chosen to spread the 8-byte line accesses of
 // the three sweep orientations over the LDS banks.
@@ -563,8 +565,9 @@ struct LdsLayout
   static constexpr int PC = (N == 4) ? 80 : (N == 6) ? 260 : (N == 7) ? 369 : (N == 8) ? 576 : N * N * N;
 };
 
-// a = L_d u along one line: tab holds the unit-interval K, d0, d1; hinv = 1/h_d. The neighbour
-// derivative traces dwnb are unit-interval derivatives as well (same h_d on a uniform box).
+// a = c L_d u along one line: tab holds the unit-interval K, d0, d1; hinv = c/h_d and tau the
+// penalties times c, c being the scale of the Kronecker term. The neighbour derivative traces
+// dwnb are unit-interval derivatives as well (same h_d on a uniform box).
 template<int N, typename Number>
 __device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, Number hinv,
                                                Number const (&u)[N], Number (&o)[N],
@@ -589,17 +592,6 @@ __device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, 
     o[i] = hinv * (o[i] + d0[i] * jl - d1[i] * jr);
   o[0] += hinv * Number(0.5) * (du0 + dwl) + Number(2) * tau[0] * jl;
   o[N - 1] += Number(2) * tau[1] * jr - hinv * Number(0.5) * (du1 + dwr);
-}
-
-// o = h M v along one line (unit-interval mass M = tab[0 .. N*N))
-template<int N, typename Number>
-__device__ __forceinline__ void mass_apply(Number const * __restrict__ M, Number h,
-                                           Number const (&v)[N], Number (&o)[N])
-{
-  mat_apply<N>(M, v, o);
-#pragma unroll
-  for(int i = 0; i < N; ++i)
-    o[i] *= h;
 }
 
 // Coordinates of the cell at traversal position pos0 + slot; the divisions act on the
@@ -774,12 +766,18 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   Number const   hx = a.h[0], hy = a.h[1], hz = a.h[2];
   Number const   hxi = a.h[3], hyi = a.h[4], hzi = a.h[5];
 
-  if constexpr(mode != ApplyMode::diagonal)
+  // Scale factors of the three Kronecker terms: with M_d = h_d M (unit-interval mass M),
+  //   out = M_z M_y (c_x L_x u) + M_z M_x (c_y L_y u) + M_y M_x (c_z L_z u),
+  // c_x = h_y h_z, c_y = h_x h_z, c_z = h_x h_y, so that every mass sweep is a plain M.
+  Number const cx = hy * hz, cy = hx * hz, cz = hx * hy;
+  // P1: the remaining neighbour traces, then the three line Laplacians into registers
+  Number rx[N], ry[N], rz[N];
+  if(active)
   {
-    if(active)
+    if constexpr(mode != ApplyMode::diagonal)
     {
-      Number w[N];
       // y traces from the staged neighbour cells, read as this lane's y-line (x = p, z = q)
+      Number w[N];
       load_line<N>(Abuf + by, L::PX, w);
       wnb[1][0]  = w[N - 1];
       dwnb[1][0] = dot_line<N>(ty + 2 * N2 + N, w);
@@ -799,79 +797,67 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
         dwnb[0][1] = dot_line<N>(tx + 2 * N2, w);
       }
     }
-    __syncthreads();  // A and B are overwritten from the X phase on
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+      Number const c = d == 0 ? cx : d == 1 ? cy : cz;
+      tau[d][0] *= c;
+      tau[d][1] *= c;
+    }
+    Number u[N];
+    load_line<N>(U + bx, 1, u);
+    line_laplacian<N>(tx, cx * hxi, u, rx, type[0], tau[0], wnb[0], dwnb[0]);
+    load_line<N>(U + by, L::PX, u);
+    line_laplacian<N>(ty, cy * hyi, u, ry, type[1], tau[1], wnb[1], dwnb[1]);
+    line_laplacian<N>(tz, cz * hzi, own, rz, type[2], tau[2], wnb[2], dwnb[2]);
   }
-
-  // X: a_x = L_x u  -> A
+  __syncthreads();  // U, A and B are read across lanes up to here
   if(active)
   {
-    Number u[N], o[N];
-    load_line<N>(U + bx, 1, u);
-    line_laplacian<N>(tx, hxi, u, o, type[0], tau[0], wnb[0], dwnb[0]);
-    store_line<N>(Abuf + bx, 1, o);
+    store_line<N>(Abuf + bx, 1, rx);
+    store_line<N>(Bbuf + by, L::PX, ry);
+    store_line<N>(U + bz, L::PY, rz);
   }
   __syncthreads();
-  // Y: A <- M_y a_x, B <- a_y = L_y u
+  // P2: B <- M_x B and U <- M_x U along x-lines, A <- M_y A along y-lines (each line in place:
+  // in one phase every line of a buffer belongs to one lane)
   if(active)
   {
-    Number u[N], o[N], v[N], w[N];
-    load_line<N>(U + by, L::PX, u);
-    line_laplacian<N>(ty, hyi, u, o, type[1], tau[1], wnb[1], dwnb[1]);
-    store_line<N>(Bbuf + by, L::PX, o);
+    Number v[N], w[N];
+    load_line<N>(Bbuf + bx, 1, v);
+    mat_apply<N>(tx, v, w);
+    store_line<N>(Bbuf + bx, 1, w);
+    load_line<N>(U + bx, 1, v);
+    mat_apply<N>(tx, v, w);
+    store_line<N>(U + bx, 1, w);
     load_line<N>(Abuf + by, L::PX, v);
-    mass_apply<N>(ty, hy, v, w);
+    mat_apply<N>(ty, v, w);
     store_line<N>(Abuf + by, L::PX, w);
   }
   __syncthreads();
-  // X2: A <- A + M_x a_y
-  if(active)
-  {
-    Number v[N], w[N], r[N];
-    load_line<N>(Bbuf + bx, 1, v);
-    mass_apply<N>(tx, hx, v, w);
-    load_line<N>(Abuf + bx, 1, r);
-#pragma unroll
-    for(int i = 0; i < N; ++i)
-      r[i] += w[i];
-    store_line<N>(Abuf + bx, 1, r);
-  }
-  __syncthreads();
-  // Z: out_partial = M_z A (registers), B <- a_z = L_z u
+  // P3: U <- M_y U along y-lines (the finished z-term); outp = M_z (A + B) along z-lines
   Number outp[N];
   if(active)
   {
-    Number o[N], v[N];
-    line_laplacian<N>(tz, hzi, own, o, type[2], tau[2], wnb[2], dwnb[2]);
+    Number v[N], w[N];
+    load_line<N>(U + by, L::PX, v);
+    mat_apply<N>(ty, v, w);
+    store_line<N>(U + by, L::PX, w);
     load_line<N>(Abuf + bz, L::PY, v);
-    mass_apply<N>(tz, hz, v, outp);
-    store_line<N>(Bbuf + bz, L::PY, o);
+    load_line<N>(Bbuf + bz, L::PY, w);
+#pragma unroll
+    for(int i = 0; i < N; ++i)
+      v[i] += w[i];
+    mat_apply<N>(tz, v, outp);
   }
   __syncthreads();
-  // X3: B <- M_x B
-  if(active)
-  {
-    Number v[N], w[N];
-    load_line<N>(Bbuf + bx, 1, v);
-    mass_apply<N>(tx, hx, v, w);
-    store_line<N>(Bbuf + bx, 1, w);
-  }
-  __syncthreads();
-  // Y3: B <- M_y B
-  if(active)
-  {
-    Number v[N], w[N];
-    load_line<N>(Bbuf + by, L::PX, v);
-    mass_apply<N>(ty, hy, v, w);
-    store_line<N>(Bbuf + by, L::PX, w);
-  }
-  __syncthreads();
-  // out = out_partial + B along the z-lines
+  // out = outp + the z-term held in U, along the z-lines
   if constexpr(mode == ApplyMode::diagonal)
   {
     if(active)
     {
       Number v[N];
-      load_line<N>(Bbuf + bz, L::PY, v);
+      load_line<N>(U + bz, L::PY, v);
 #pragma unroll
       for(int k = 0; k < N; ++k)
         if(lt + k * N2 == a.probe)
@@ -883,7 +869,7 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
     // direct per-lane stores: lane (p, q) owns the z-line of DoFs lt + k N2 (a workgroup-wide
     // LDS-staged store of the block's contiguous cells measured 10% slower)
     Number v[N];
-    load_line<N>(Bbuf + bz, L::PY, v);
+    load_line<N>(U + bz, L::PY, v);
     size_t const g0 = (size_t)cell * N3 + lt;
 #pragma unroll
     for(int k = 0; k < N; ++k)
