@@ -18,15 +18,34 @@ enum class ApplyMode : int
   chebyshev = 3
 };
 
+// Centrosymmetric n x n matrices (A_ij = A_{n-1-i,n-1-j}, such as the 1D mass and stiffness
+// on symmetric nodes) are stored for n >= 5 in even-odd form, which needs ~n^2/2 products per
+// application instead of n^2: with m = n/2 and m_e = m + n%2,
+//   Ae (m x m_e): (A_ij + A_{i,n-1-j})/2 for j < m, A_im for j = m (odd n),
+//   Ao (m x m):   (A_ij - A_{i,n-1-j})/2,
+//   Am (m_e, odd n only): the middle row A_mj, j <= m.
+constexpr bool cs_packed(int n) { return n >= 5; }
+constexpr int  cs_size(int n)
+{
+  return cs_packed(n) ? (n / 2) * (n / 2 + n % 2) + (n / 2) * (n / 2) + (n % 2 ? n / 2 + 1 : 0)
+                      : n * n;
+}
+// Offsets in the Cartesian kernel's table: [M | K | d0 n | d1 n], M and K of cs_size(n) each
+template<int N>
+struct CartTable
+{
+  static constexpr int M = 0, K = cs_size(N), D0 = 2 * cs_size(N), D1 = D0 + N;
+};
+
 // Table layout (Number, SipgArgs::tab): general kernel: [S n*n | Dq n*n | d0 n | d1 n];
-// Cartesian kernel: [M n*n | K n*n | d0 n | d1 n] on the unit interval plus h per direction.
+// Cartesian kernel: CartTable, on the unit interval, plus h per direction.
 template<typename Number>
 struct SipgArgs
 {
-  // 1D tables [A n*n | B n*n | c n | e n]. The Cartesian kernel reads them from this by-value
-  // copy, i.e. as uniform scalar kernel-argument loads (unit-interval mass M, stiffness K,
-  // end derivatives d0, d1, scaled per direction by h or 1/h inside the kernel); the general
-  // kernel (S, Dq, d0, d1) indexes them at run time and reads the device copy `tables`.
+  // 1D tables. The Cartesian kernel reads them from this by-value copy, i.e. as uniform scalar
+  // kernel-argument loads (unit-interval mass M, stiffness K, end derivatives d0, d1, scaled
+  // per direction inside the kernel); the general kernel (S, Dq, d0, d1) indexes them at run
+  // time and reads the device copy `tables`.
   static constexpr int kMaxTable = 2 * 9 * 9 + 2 * 9;
   Number         tab[kMaxTable];
   Number         h[6];  // Cartesian kernel: h_x, h_y, h_z, 1/h_x, 1/h_y, 1/h_z

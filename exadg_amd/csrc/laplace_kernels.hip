@@ -40,6 +40,52 @@ __device__ __forceinline__ void mat_apply(Number const * __restrict__ M, Number 
   }
 }
 
+// out = A in for a centrosymmetric A stored as CartTable expects (even-odd form for n >= 5)
+template<int N, typename Number>
+__device__ __forceinline__ void cs_apply(Number const * __restrict__ T, Number const (&in)[N],
+                                         Number (&out)[N])
+{
+  if constexpr(!cs_packed(N))
+    mat_apply<N>(T, in, out);
+  else
+  {
+    constexpr int  m = N / 2, me = m + N % 2;
+    Number const * Ae = T;
+    Number const * Ao = T + m * me;
+    Number const * Am = Ao + m * m;
+    Number         e[me], o[m];
+#pragma unroll
+    for(int j = 0; j < m; ++j)
+    {
+      e[j] = in[j] + in[N - 1 - j];
+      o[j] = in[j] - in[N - 1 - j];
+    }
+    if constexpr(N % 2 == 1)
+      e[m] = in[m];
+#pragma unroll
+    for(int i = 0; i < m; ++i)
+    {
+      Number E = 0, O = 0;
+#pragma unroll
+      for(int j = 0; j < me; ++j)
+        E += Ae[i * me + j] * e[j];
+#pragma unroll
+      for(int j = 0; j < m; ++j)
+        O += Ao[i * m + j] * o[j];
+      out[i]         = E + O;
+      out[N - 1 - i] = E - O;
+    }
+    if constexpr(N % 2 == 1)
+    {
+      Number s = 0;
+#pragma unroll
+      for(int j = 0; j < me; ++j)
+        s += Am[j] * e[j];
+      out[m] = s;
+    }
+  }
+}
+
 // out[j] = sum_i M[i*N+j] in[i]
 template<int N, typename Number>
 __device__ __forceinline__ void mat_apply_T(Number const * __restrict__ M, Number const (&in)[N],
@@ -574,10 +620,10 @@ __device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, 
                                                int const (&type)[2], Number const (&tau)[2],
                                                Number const (&wnb)[2], Number const (&dwnb)[2])
 {
-  Number const * K  = tab + N * N;
-  Number const * d0 = tab + 2 * N * N;
-  Number const * d1 = d0 + N;
-  mat_apply<N>(K, u, o);
+  Number const * K  = tab + CartTable<N>::K;
+  Number const * d0 = tab + CartTable<N>::D0;
+  Number const * d1 = tab + CartTable<N>::D1;
+  cs_apply<N>(K, u, o);
   Number const du0 = dot_line<N>(d0, u), du1 = dot_line<N>(d1, u);
   // exterior traces: neighbour values or mirror values
   Number const sl  = type[0] == EXADG_BC_DIRICHLET ? Number(-1) : Number(1);
@@ -699,9 +745,9 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
           w[i] = (gbase + i * gstr == a.probe) ? Number(1) : Number(0);
         bool const self = nb[d][0] == cell;
         wnb[d][0]       = self ? w[N - 1] : Number(0);
-        dwnb[d][0]      = self ? dot_line<N>(tb + 2 * N2 + N, w) : Number(0);
+        dwnb[d][0]      = self ? dot_line<N>(tb + CartTable<N>::D1, w) : Number(0);
         wnb[d][1]       = self ? w[0] : Number(0);
-        dwnb[d][1]      = self ? dot_line<N>(tb + 2 * N2, w) : Number(0);
+        dwnb[d][1]      = self ? dot_line<N>(tb + CartTable<N>::D0, w) : Number(0);
       }
     }
     else
@@ -750,13 +796,13 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
       Number const * tzb = a.tab;
       Number const * txb = a.tab;
       wnb[2][0]          = zl[N - 1];
-      dwnb[2][0]         = dot_line<N>(tzb + 2 * N2 + N, zl);
+      dwnb[2][0]         = dot_line<N>(tzb + CartTable<N>::D1, zl);
       wnb[2][1]          = zr[0];
-      dwnb[2][1]         = dot_line<N>(tzb + 2 * N2, zr);
+      dwnb[2][1]         = dot_line<N>(tzb + CartTable<N>::D0, zr);
       wnb[0][0]          = xl[N - 1];
-      dwnb[0][0]         = dot_line<N>(txb + 2 * N2 + N, xl);
+      dwnb[0][0]         = dot_line<N>(txb + CartTable<N>::D1, xl);
       wnb[0][1]          = xr[0];
-      dwnb[0][1]         = dot_line<N>(txb + 2 * N2, xr);
+      dwnb[0][1]         = dot_line<N>(txb + CartTable<N>::D0, xr);
     }
   }
   __syncthreads();
@@ -780,21 +826,21 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
       Number w[N];
       load_line<N>(Abuf + by, L::PX, w);
       wnb[1][0]  = w[N - 1];
-      dwnb[1][0] = dot_line<N>(ty + 2 * N2 + N, w);
+      dwnb[1][0] = dot_line<N>(ty + CartTable<N>::D1, w);
       load_line<N>(Bbuf + by, L::PX, w);
       wnb[1][1]  = w[0];
-      dwnb[1][1] = dot_line<N>(ty + 2 * N2, w);
+      dwnb[1][1] = dot_line<N>(ty + CartTable<N>::D0, w);
       if(in_block_l)
       {
         load_line<N>(U - 3 * L::PC + bx, 1, w);
         wnb[0][0]  = w[N - 1];
-        dwnb[0][0] = dot_line<N>(tx + 2 * N2 + N, w);
+        dwnb[0][0] = dot_line<N>(tx + CartTable<N>::D1, w);
       }
       if(in_block_r)
       {
         load_line<N>(U + 3 * L::PC + bx, 1, w);
         wnb[0][1]  = w[0];
-        dwnb[0][1] = dot_line<N>(tx + 2 * N2, w);
+        dwnb[0][1] = dot_line<N>(tx + CartTable<N>::D0, w);
       }
     }
 #pragma unroll
@@ -825,13 +871,13 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   {
     Number v[N], w[N];
     load_line<N>(Bbuf + bx, 1, v);
-    mat_apply<N>(tx, v, w);
+    cs_apply<N>(tx + CartTable<N>::M, v, w);
     store_line<N>(Bbuf + bx, 1, w);
     load_line<N>(U + bx, 1, v);
-    mat_apply<N>(tx, v, w);
+    cs_apply<N>(tx + CartTable<N>::M, v, w);
     store_line<N>(U + bx, 1, w);
     load_line<N>(Abuf + by, L::PX, v);
-    mat_apply<N>(ty, v, w);
+    cs_apply<N>(ty + CartTable<N>::M, v, w);
     store_line<N>(Abuf + by, L::PX, w);
   }
   __syncthreads();
@@ -841,14 +887,14 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
   {
     Number v[N], w[N];
     load_line<N>(U + by, L::PX, v);
-    mat_apply<N>(ty, v, w);
+    cs_apply<N>(ty + CartTable<N>::M, v, w);
     store_line<N>(U + by, L::PX, w);
     load_line<N>(Abuf + bz, L::PY, v);
     load_line<N>(Bbuf + bz, L::PY, w);
 #pragma unroll
     for(int i = 0; i < N; ++i)
       v[i] += w[i];
-    mat_apply<N>(tz, v, outp);
+    cs_apply<N>(tz + CartTable<N>::M, v, outp);
   }
   __syncthreads();
   // out = outp + the z-term held in U, along the z-lines

@@ -8,6 +8,39 @@ namespace ExaDG
 {
 namespace Poisson
 {
+namespace
+{
+// The even-odd form of a centrosymmetric n x n matrix (row-major) that the Cartesian kernel
+// reads (CartTable / cs_size); the plain matrix for n < 5. Throws if A is not
+// centrosymmetric, which it is for the 1D mass and stiffness on symmetric nodes.
+std::vector<double> centrosymmetric_pack(std::vector<double> const & A, int n)
+{
+  if(!cs_packed(n))
+    return A;
+  double amax = 0;
+  for(double v : A)
+    amax = std::max(amax, std::abs(v));
+  for(int i = 0; i < n; ++i)
+    for(int j = 0; j < n; ++j)
+      if(std::abs(A[i * n + j] - A[(n - 1 - i) * n + (n - 1 - j)]) > 1e-12 * amax)
+        EXADG_THROW(EXADG_ERR_INTERNAL, "1D matrix is not centrosymmetric");
+  int const           m = n / 2, me = m + n % 2;
+  std::vector<double> t;
+  for(int i = 0; i < m; ++i)
+    for(int j = 0; j < me; ++j)
+      t.push_back(j < m ? 0.5 * (A[i * n + j] + A[i * n + n - 1 - j]) : A[i * n + m]);
+  for(int i = 0; i < m; ++i)
+    for(int j = 0; j < m; ++j)
+      t.push_back(0.5 * (A[i * n + j] - A[i * n + n - 1 - j]));
+  if(n % 2)
+    for(int j = 0; j <= m; ++j)
+      t.push_back(A[m * n + j]);
+  if((int)t.size() != cs_size(n))
+    EXADG_THROW(EXADG_ERR_INTERNAL, "even-odd table size");
+  return t;
+}
+}  // namespace
+
 template<typename Number>
 LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> grid,
                                          LaplaceOperatorData const &           data)
@@ -21,25 +54,32 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
   bool const curved = !grid_->affine();
   geo_              = compute_geometry(*grid_, shape_, data.IP_factor, curved);
 
-  // 1D tables travel by value in the kernel arguments (uniform scalar loads)
-  std::vector<double> tab;
-  auto append = [&tab](std::vector<double> const & v) { tab.insert(tab.end(), v.begin(), v.end()); };
+  // 1D tables travel by value in the kernel arguments (uniform scalar loads); the general
+  // kernel reads the plain layout from device memory
+  std::vector<double> tab, plain;
+  auto append = [](std::vector<double> & t, std::vector<double> const & v) {
+    t.insert(t.end(), v.begin(), v.end());
+  };
   if(curved)
   {
-    append(shape_.S);
-    append(shape_.Dq);
+    append(tab, shape_.S);
+    append(tab, shape_.Dq);
   }
   else
   {
-    append(shape_.mass);
-    append(shape_.stiff);
+    append(tab, centrosymmetric_pack(shape_.mass, n));
+    append(tab, centrosymmetric_pack(shape_.stiff, n));
   }
-  append(shape_.d0);
-  append(shape_.d1);
+  append(tab, shape_.d0);
+  append(tab, shape_.d1);
+  append(plain, curved ? shape_.S : shape_.mass);
+  append(plain, curved ? shape_.Dq : shape_.stiff);
+  append(plain, shape_.d0);
+  append(plain, shape_.d1);
   if((int)tab.size() > SipgArgs<Number>::kMaxTable)
     EXADG_THROW(EXADG_ERR_INTERNAL, "1D tables exceed the kernel-argument table");
   tab_.assign(tab.begin(), tab.end());
-  tables_.upload(tab_);
+  tables_.upload(std::vector<Number>(plain.begin(), plain.end()));
   for(int d = 0; d < 3; ++d)
   {
     h_[d]     = (Number)grid_->h(d);
