@@ -178,7 +178,9 @@ __device__ __forceinline__ void source_line(Number const * __restrict__ src, int
 template<int N>
 constexpr int general_work_size()
 {
-  return (3 * N * N * N > 16 * N * N) ? 3 * N * N * N : 16 * N * N;
+  // cell phase: the x and y gradients (the z one stays in registers); face phases: the 8 trace
+  // arrays, which the flux coefficients then overwrite
+  return (2 * N * N * N > 8 * N * N) ? 2 * N * N * N : 8 * N * N;
 }
 
 template<int N, typename Number, ApplyMode mode>
@@ -193,8 +195,8 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
   Number const * Dq = a.tables + N2;
   Number const * d0 = a.tables + 2 * N2;
   Number const * d1 = d0 + N;
-  Number *       E  = W;           // E[(s*4+c)*N2]: c = u-, du-/dxi_d, u+, du+/dxi_d
-  Number *       R  = W + 8 * N2;  // R[(s*4+c)*N2]: c = value test, t1, t2, d test coefficients
+  Number *       E = W;  // E[(s*4+c)*N2]: c = u-, du-/dxi_d, u+, du+/dxi_d
+  Number *       R = W;  // after F4, over E: R[(s*4+c)*N2]: c = value, t1, t2, d test coefficients
   int const      t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
   int const      p = lt % N, q = lt / N;
 
@@ -267,15 +269,17 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
       store_line<N>(arr + r, N, o);
     }
   __syncthreads();
-  // F4: fluxes at the 2 x N^2 face quadrature points
+  // F4: fluxes at the 2 x N^2 face quadrature points (task s*N2 + lt: side s, point lt), kept
+  // in registers until every lane has read E, then written over it
+  Number res[2][4];
   if(active)
-    for(int task = lt; task < 2 * N2; task += N2)
+#pragma unroll
+    for(int s = 0; s < 2; ++s)
     {
-      int const s = task / N2, qq = task % N2, q1 = qq % N, q2 = qq / N;
-      Number *  Rs = R + s * 4 * N2;
+      int const qq = lt, q1 = qq % N, q2 = qq / N;
       if(type[s] == EXADG_BC_NEUMANN)
       {  // homogeneous Neumann: u+ = u-, du+/dn = -du-/dn -> no contribution
-        Rs[qq] = Rs[N2 + qq] = Rs[2 * N2 + qq] = Rs[3 * N2 + qq] = Number(0);
+        res[s][0] = res[s][1] = res[s][2] = res[s][3] = Number(0);
         continue;
       }
       int64_t const  f  = face_id(a.topo, cc, d, s);
@@ -320,12 +324,18 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
       Number const tau   = a.tau_face[f];
       Number const jump  = u - uN;
       Number const sigma = Number(-0.5) * jump * w;
-      Number const mu    = -(Number(0.5) * (dudn_K + dudn_N) - tau * jump) * w;
-      Rs[qq]             = mu;
-      Rs[N2 + qq]        = sigma * aK[t1];
-      Rs[2 * N2 + qq]    = sigma * aK[t2];
-      Rs[3 * N2 + qq]    = sigma * aK[d];
+      res[s][0]          = -(Number(0.5) * (dudn_K + dudn_N) - tau * jump) * w;
+      res[s][1]          = sigma * aK[t1];
+      res[s][2]          = sigma * aK[t2];
+      res[s][3]          = sigma * aK[d];
     }
+  __syncthreads();
+  if(active)
+#pragma unroll
+    for(int s = 0; s < 2; ++s)
+#pragma unroll
+      for(int c = 0; c < 4; ++c)
+        R[(s * 4 + c) * N2 + lt] = res[s][c];
   __syncthreads();
   // F5: tangential gradient test functions folded into the value test (Dq^T)
   if(active)
@@ -388,8 +398,17 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
   __syncthreads();
 }
 
+// Minimum waves per SIMD asked of the compiler for the general kernel: at n = 6 in FP64 the
+// LDS allows four workgroups per CU, and the register allocation is held to 128 VGPRs to match
+template<int N, typename Number>
+constexpr int general_min_waves()
+{
+  return (N == 6 && sizeof(Number) == 8) ? 4 : 1;
+}
+
 template<int N, typename Number, ApplyMode mode>
-__global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
+__global__ void __launch_bounds__(256)
+  __attribute__((amdgpu_waves_per_eu(general_min_waves<N, Number>()))) sipg_general_kernel(SipgArgs<Number> a)
 {
   constexpr int  N2 = N * N, N3 = N * N * N, WS = general_work_size<N>();
   constexpr int  CPB = cells_per_block(N);
@@ -402,7 +421,7 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
   int64_t const  cell   = active ? traversal_cell(a.topo, pos, a.tile_rows) : 0;
   Number *       A      = smem + (active ? slot : 0) * (N3 + WS);
   Number *       W      = A + N3;
-  Number *       Gx = W, *Gy = W + N3, *Gz = W + 2 * N3;
+  Number *       Gx = W, *Gy = W + N3;
   Number const * S  = a.tables;
   Number const * Dq = a.tables + N2;
   int const      p = lt % N, q = lt / N;
@@ -466,15 +485,16 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
     store_line<N>(A + p + N2 * q, N, o);
   }
   __syncthreads();
-  // z sweep, then the z derivative at the Gauss points
+  // z sweep, then the z derivative at the Gauss points: this lane's z-line (points lt + m N2),
+  // kept in registers -- the same lane applies the metric at those points below
+  Number gz_line[N];
   if(active)
   {
-    Number v[N], o[N], g[N];
+    Number v[N], o[N];
     load_line<N>(A + p + N * q, N2, v);
     mat_apply<N>(S, v, o);
     store_line<N>(A + p + N * q, N2, o);
-    mat_apply<N>(Dq, o, g);
-    store_line<N>(Gz + p + N * q, N2, g);
+    mat_apply<N>(Dq, o, gz_line);
   }
   __syncthreads();
   if(active)
@@ -496,12 +516,12 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
     for(int m = 0; m < N; ++m)
     {
       int const    e  = lt + m * N2;
-      Number const gx = Gx[e], gy = Gy[e], gz = Gz[e];
+      Number const gx = Gx[e], gy = Gy[e], gz = gz_line[m];
       Number const g00 = G[e], g11 = G[N3 + e], g22 = G[2 * N3 + e];
       Number const g01 = G[3 * N3 + e], g02 = G[4 * N3 + e], g12 = G[5 * N3 + e];
       Gx[e]            = g00 * gx + g01 * gy + g02 * gz;
       Gy[e]            = g01 * gx + g11 * gy + g12 * gz;
-      Gz[e]            = g02 * gx + g12 * gy + g22 * gz;
+      gz_line[m]       = g02 * gx + g12 * gy + g22 * gz;
     }
   }
   __syncthreads();
@@ -528,9 +548,8 @@ __global__ void __launch_bounds__(256) sipg_general_kernel(SipgArgs<Number> a)
   __syncthreads();
   if(active)
   {
-    Number v[N], o[N], w[N];
-    load_line<N>(Gz + p + N * q, N2, v);
-    mat_apply_T<N>(Dq, v, o);
+    Number o[N], w[N];
+    mat_apply_T<N>(Dq, gz_line, o);
     load_line<N>(A + p + N * q, N2, w);
 #pragma unroll
     for(int i = 0; i < N; ++i)
