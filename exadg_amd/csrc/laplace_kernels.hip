@@ -416,9 +416,8 @@ __global__ void __launch_bounds__(256)
   Number *       smem = reinterpret_cast<Number *>(smem_raw);
   int const      slot = threadIdx.x / N2;
   int const      lt   = threadIdx.x - slot * N2;
-  int64_t const  pos  = xcd_logical_block(blockIdx.x, gridDim.x) * CPB + slot;
-  bool const     active = slot < CPB && pos < a.n_cells;
-  int64_t const  cell   = active ? traversal_cell(a.topo, pos, a.tile_rows) : 0;
+  int const      pos0 = (int)xcd_logical_block(blockIdx.x, gridDim.x) * CPB;
+  bool const     active = slot < CPB && pos0 + slot < (int)a.n_cells;
   Number *       A      = smem + (active ? slot : 0) * (N3 + WS);
   Number *       W      = A + N3;
   Number *       Gx = W, *Gy = W + N3;
@@ -427,7 +426,8 @@ __global__ void __launch_bounds__(256)
   int const      p = lt % N, q = lt / N;
   int            cc[3] = {0, 0, 0};
   if(active)
-    cell_coords(a.topo, cell, cc);
+    traversal_coords(a.topo, a.tile_rows, pos0, slot, cc);
+  int64_t const cell = active ? cell_id(a.topo, cc) : 0;
 
   // neighbour traces of this lane's line in every direction: value and derivative at the
   // shared face, read up front so that the loads overlap the cell phases
@@ -657,40 +657,6 @@ __device__ __forceinline__ void line_laplacian(Number const * __restrict__ tab, 
     o[i] = hinv * (o[i] + d0[i] * jl - d1[i] * jr);
   o[0] += hinv * Number(0.5) * (du0 + dwl) + Number(2) * tau[0] * jl;
   o[N - 1] += Number(2) * tau[1] * jr - hinv * Number(0.5) * (du1 + dwr);
-}
-
-// Coordinates of the cell at traversal position pos0 + slot; the divisions act on the
-// block-uniform pos0 only, per lane there is at most a short carry along x.
-__device__ __forceinline__ void traversal_coords(BoxTopology const & t, int tile_rows, int pos0,
-                                                 int slot, int (&cc)[3])
-{
-  int const nx = t.nc[0], ny = t.nc[1], nz = t.nc[2];
-  int const full = nx * nz * tile_rows;
-  int       tile = pos0 / full;
-  int const r    = pos0 - tile * full;
-  int       rows = min(tile_rows, ny - tile * tile_rows);
-  int       x    = r % nx;
-  int const r2   = r / nx;
-  int       yi   = r2 % rows;
-  int       z    = r2 / rows;
-  x += slot;
-  while(x >= nx)
-  {
-    x -= nx;
-    if(++yi == rows)
-    {
-      yi = 0;
-      if(++z == nz)
-      {
-        z = 0;
-        ++tile;
-        rows = min(tile_rows, ny - tile * tile_rows);
-      }
-    }
-  }
-  cc[0] = x;
-  cc[1] = tile * tile_rows + yi;
-  cc[2] = z;
 }
 
 template<int N, typename Number, ApplyMode mode>

@@ -44,7 +44,7 @@ __host__ __device__ inline int64_t neighbor_id(BoxTopology const & t, int const 
   {
     if(!t.periodic[d])
       return -1;
-    n[d] = (n[d] + t.nc[d]) % t.nc[d];
+    n[d] += n[d] < 0 ? t.nc[d] : -t.nc[d];  // one step across the periodic seam
   }
   return cell_id(t, n);
 }
@@ -74,6 +74,41 @@ __host__ __device__ inline int64_t traversal_cell(BoxTopology const & t, int64_t
   int const     yi   = (int)(r2 % rows);
   int const     z    = (int)(r2 / rows);
   return x + (int64_t)nx * ((int64_t)tile * tile_rows + yi + (int64_t)ny * z);
+}
+
+// Coordinates of the cell at traversal position pos0 + slot (the order of traversal_cell). The
+// divisions act on pos0 only, which the kernels keep uniform over a workgroup (scalar
+// arithmetic); per lane there is at most a short carry along x.
+__host__ __device__ inline void traversal_coords(BoxTopology const & t, int tile_rows, int pos0,
+                                                 int slot, int (&cc)[3])
+{
+  int const nx = t.nc[0], ny = t.nc[1], nz = t.nc[2];
+  int const full = nx * nz * tile_rows;
+  int       tile = pos0 / full;
+  int const r    = pos0 - tile * full;
+  int       rows = ny - tile * tile_rows < tile_rows ? ny - tile * tile_rows : tile_rows;
+  int       x    = r % nx;
+  int const r2   = r / nx;
+  int       yi   = r2 % rows;
+  int       z    = r2 / rows;
+  x += slot;
+  while(x >= nx)
+  {
+    x -= nx;
+    if(++yi == rows)
+    {
+      yi = 0;
+      if(++z == nz)
+      {
+        z = 0;
+        ++tile;
+        rows = ny - tile * tile_rows < tile_rows ? ny - tile * tile_rows : tile_rows;
+      }
+    }
+  }
+  cc[0] = x;
+  cc[1] = tile * tile_rows + yi;
+  cc[2] = z;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (block b runs on XCD
