@@ -14,7 +14,12 @@ namespace
 {
 constexpr int TM = 64, TN = 64, TK = 16;
 
-// C(i,j) = alpha * sum_k A(i,k) B(k,j) (+ C) (+ bias_j), optional ReLU.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// C(i,j) = alpha * sum_k A(i,k) B(k,j) (+ C) (+ bias_j), optional ReLU, on the matrix cores:
+// each of the 4 waves computes a 32 x 32 quadrant of the 64 x 64 tile as 2 x 2 products
+// v_mfma_f32_16x16x4f32 (FP32 operands, FP32 accumulation). Lane l supplies A(l%16, l/16) and
+// B(l/16, l%16) of a 16 x 4 / 4 x 16 step and holds D(4 (l/16) + r, l%16), r = 0..3.
 // A(i,k) = A[i*a_i + k*a_k], B(k,j) = B[k*b_k + j*b_j], C row-major with leading dim ldc.
 // With gridDim.z > 1 (split K), block z covers k in [z*kc, min(K, (z+1)*kc)) and writes its
 // raw partial product to C + z * M * ldc; splitk_reduce_kernel applies alpha/C/bias/ReLU.
@@ -27,7 +32,6 @@ __global__ void __launch_bounds__(256)
   __shared__ float As[TK][TM + 1];
   __shared__ float Bs[TK][TN + 1];
   int const tid = threadIdx.x;
-  int const tx = tid % 16, ty = tid / 16;
   int const i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
   int const kb = blockIdx.z * kc, ke = min(K, kb + kc);
   bool const split = gridDim.z > 1;
@@ -36,7 +40,15 @@ __global__ void __launch_bounds__(256)
     C += (int64_t)blockIdx.z * M * ldc;
     alpha = 1.f, accumulate = 0, bias = nullptr, relu = 0;
   }
-  float acc[4][4] = {};
+  int const lane = tid & 63, wave = tid >> 6;
+  int const wr = 32 * (wave >> 1), wc = 32 * (wave & 1);  // this wave's quadrant
+  int const li = lane & 15, lk = lane >> 4;
+  f32x4     acc[2][2];
+#pragma unroll
+  for(int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for(int cb = 0; cb < 2; ++cb)
+      acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
   for(int k0 = kb; k0 < ke; k0 += TK)
   {
     // 64x16 tiles of A and 16x64 of B: 4 elements per thread each
@@ -54,45 +66,43 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
 #pragma unroll
-    for(int kk = 0; kk < TK; ++kk)
+    for(int kk = 0; kk < TK; kk += 4)
     {
-      float a[4], b[4];
+      float a[2], b[2];
 #pragma unroll
-      for(int r = 0; r < 4; ++r)
-        a[r] = As[kk][ty * 4 + r];
+      for(int rb = 0; rb < 2; ++rb)
+        a[rb] = As[kk + lk][wr + 16 * rb + li];
 #pragma unroll
-      for(int c = 0; c < 4; ++c)
-        b[c] = Bs[kk][tx * 4 + c];
+      for(int cb = 0; cb < 2; ++cb)
+        b[cb] = Bs[kk + lk][wc + 16 * cb + li];
 #pragma unroll
-      for(int r = 0; r < 4; ++r)
+      for(int rb = 0; rb < 2; ++rb)
 #pragma unroll
-        for(int c = 0; c < 4; ++c)
-          acc[r][c] += a[r] * b[c];
+        for(int cb = 0; cb < 2; ++cb)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for(int r = 0; r < 4; ++r)
-  {
-    int const i = i0 + ty * 4 + r;
-    if(i >= M)
-      continue;
+  for(int rb = 0; rb < 2; ++rb)
 #pragma unroll
-    for(int c = 0; c < 4; ++c)
-    {
-      int const j = j0 + tx * 4 + c;
-      if(j >= N)
-        continue;
-      float v = alpha * acc[r][c];
-      if(accumulate)
-        v += C[i * ldc + j];
-      if(bias)
-        v += bias[j];
-      if(relu)
-        v = v > 0.f ? v : 0.f;
-      C[i * ldc + j] = v;
-    }
-  }
+    for(int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for(int r = 0; r < 4; ++r)
+      {
+        int const i = i0 + wr + 16 * rb + 4 * lk + r;
+        int const j = j0 + wc + 16 * cb + li;
+        if(i >= M || j >= N)
+          continue;
+        float v = alpha * acc[rb][cb][r];
+        if(accumulate)
+          v += C[i * ldc + j];
+        if(bias)
+          v += bias[j];
+        if(relu)
+          v = v > 0.f ? v : 0.f;
+        C[i * ldc + j] = v;
+      }
 }
 
 __global__ void __launch_bounds__(256)

@@ -114,16 +114,25 @@ def ab_test(ex, torch, op, prec, net, P, Bv, s_begin, s_end, rel_tol=1e-10,
     return out
 
 
-def train(ex, net, torch, op, P, Bv, t_end, epochs, lr, loss=None, batch_cells=4096, seed=7):
+def train(ex, net, torch, op, P, Bv, t_end, epochs, lr, loss=None, batch_cells=4096, seed=7,
+          halve_every=5):
     """Train on steps [0, t_end) of the FP64 sequence; one call per epoch so that the per-epoch
-    losses are returned."""
+    losses are returned. The learning rate halves every `halve_every` epochs, and the
+    parameters after the epoch with the lowest training loss are kept: Adam on the residual
+    loss can collapse late in training (the network output falling to zero, loss 1) after a
+    good early epoch. Only training steps are looked at."""
     if loss is None:
         loss = ex.GUESS_LOSS_ENERGY
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    losses = []
+    losses, best, best_params = [], float("inf"), None
     for ep in range(epochs):
-        losses.append(net.train(P, Bv, 0, t_end, 1, op=op, loss=loss, batch_cells=batch_cells, lr=lr,
-                                seed=seed + ep))
+        rate = lr * 0.5 ** (ep // halve_every)
+        losses.append(net.train(P, Bv, 0, t_end, 1, op=op, loss=loss, batch_cells=batch_cells,
+                                lr=rate, seed=seed + ep))
+        if losses[-1] < best:
+            best, best_params = losses[-1], net.params()
+    if best_params is not None and not losses[-1] <= best:
+        net.set_params(best_params)
     torch.cuda.synchronize()
     return losses, time.perf_counter() - t0
