@@ -36,3 +36,26 @@ def test_error_path_without_gpu():
     assert L.exadg_laplace_create(None, ctypes.byref(b), 2, 0, 1.0, ctypes.byref(h)) == 1
     bad = exadg_amd.mesh_desc((0, 1, 1))
     assert L.exadg_laplace_create(ctypes.byref(bad), ctypes.byref(b), 2, 0, 1.0, ctypes.byref(h)) == 1
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built HIP library the package raises; there is no CPU fallback to route to."""
+    import pytest
+    monkeypatch.setattr(exadg_amd, "_lib", None)
+    monkeypatch.setattr(exadg_amd, "LIB_PATH", str(tmp_path / "libexadg_amd.so"))
+    with pytest.raises(ImportError, match="build it"):
+        exadg_amd.lib()
+
+
+def test_product_does_not_use_the_oracle():
+    """The product package and its native sources neither import nor link the oracle."""
+    root = os.path.dirname(os.path.abspath(exadg_amd.__file__))
+    offenders = []
+    for dirpath, _, files in os.walk(root):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h")):
+                text = open(os.path.join(dirpath, f), errors="replace").read()
+                if re.search(r"^\s*(import|from)\s+oracle\b|liboracle|oracle_capi|oracle_core",
+                             text, flags=re.M):
+                    offenders.append(f)
+    assert not offenders, offenders
