@@ -399,13 +399,14 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
 }
 
 // Minimum waves per SIMD asked of the compiler for the general kernel, matched to what the
-// LDS allows: FP64 at n = 6 four (128 VGPRs, a 12-byte spill, measured 8% faster than three);
-// FP32 at n = 4..6 seven (<= 72 VGPRs, no spill, 5-6% faster than six) and at n = 7 six
+// LDS allows: FP64 at n = 6 four (128 VGPRs, a 12-byte spill, measured 8% faster than three)
+// and at n = 3 five (96 VGPRs, no spill, 5% faster than four; at n = 4 five spill and are
+// slower); FP32 at n = 4..6 seven (<= 72 VGPRs, no spill, 5-6% faster than six), at n = 7 six
 template<int N, typename Number>
 constexpr int general_min_waves()
 {
   if constexpr(sizeof(Number) == 8)
-    return N == 6 ? 4 : 1;
+    return N == 6 ? 4 : N == 3 ? 5 : 1;
   else
     return (N >= 4 && N <= 6) ? 7 : N == 7 ? 6 : 1;
 }
