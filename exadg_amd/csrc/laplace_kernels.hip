@@ -625,14 +625,15 @@ __global__ void __launch_bounds__(256)
 // (x-lines) and M_y a_x (y-lines); M_y M_x a_z (y-lines) and M_z (...) (z-lines); the sum along
 // the z-lines. The neighbour lines are read once, at kernel start, and reduced to their traces.
 
-// Padded LDS strides (x, y, z, cell) per n, chosen to spread the 8-byte line accesses of
-// the three sweep orientations over the LDS banks.
+// LDS strides (x, y, z, cell) per n. n = 4 and 7 are padded to spread the 8-byte line
+// accesses of the three sweep orientations over the LDS banks; at n = 6 and 8 padding costs a
+// workgroup per CU in FP64 (measured 6-10% slower), so those are dense.
 template<int N>
 struct LdsLayout
 {
-  static constexpr int PX = (N == 8) ? 9 : N;
-  static constexpr int PY = (N == 4) ? 19 : (N == 6) ? 39 : (N == 7) ? 52 : (N == 8) ? 72 : N * N;
-  static constexpr int PC = (N == 4) ? 80 : (N == 6) ? 260 : (N == 7) ? 369 : (N == 8) ? 576 : N * N * N;
+  static constexpr int PX = N;
+  static constexpr int PY = (N == 4) ? 19 : (N == 7) ? 52 : N * N;
+  static constexpr int PC = (N == 4) ? 80 : (N == 7) ? 369 : N * N * N;
 };
 
 // a = c L_d u along one line: tab holds the unit-interval K, d0, d1; hinv = c/h_d and tau the
