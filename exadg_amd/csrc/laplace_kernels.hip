@@ -175,83 +175,98 @@ __device__ __forceinline__ void source_line(Number const * __restrict__ src, int
 // normal-derivative vectors a = J^{-1} n of both sides, transposed back and added to the
 // own lines.
 
-template<int N>
-constexpr int general_work_size()
+// Face directions handled per round of face phases: two (x and y together, then z) in FP32,
+// where the LDS has room for their 16 n^2 face arrays at full occupancy; one in FP64
+template<typename Number>
+constexpr int face_dirs_per_round()
 {
-  // cell phase: the x and y gradients (the z one stays in registers); face phases: the 8 trace
-  // arrays, which the flux coefficients then overwrite
-  return (2 * N * N * N > 8 * N * N) ? 2 * N * N * N : 8 * N * N;
+  return sizeof(Number) == 4 ? 2 : 1;
 }
 
-template<int N, typename Number, ApplyMode mode>
-__device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> const & a,
-                                                       int64_t cell, int const * cc, bool active,
-                                                       int lt, Number * A, Number * W,
-                                                       Number const (&nbw)[2],
-                                                       Number const (&nbdw)[2])
+template<int N, typename Number>
+constexpr int general_work_size()
+{
+  // cell phase: the x and y gradients (the z one stays in registers); face phases: 8 trace
+  // arrays per direction of a round, which the flux coefficients then overwrite
+  constexpr int faces = 8 * face_dirs_per_round<Number>() * N * N;
+  return (2 * N * N * N > faces) ? 2 * N * N * N : faces;
+}
+
+// Face terms of the ND directions dirs[] (compile-time after unrolling) in one round of
+// barrier-separated steps; the face arrays of direction j live at W + 8 j N^2.
+template<int N, typename Number, ApplyMode mode, int ND>
+__device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Number> const & a,
+                                              int64_t cell, int const * cc, bool active, int lt,
+                                              Number * A, Number * W,
+                                              Number const (&nbw)[3][2],
+                                              Number const (&nbdw)[3][2])
 {
   constexpr int  N2 = N * N, N3 = N * N * N;
   Number const * S  = a.tables;
   Number const * Dq = a.tables + N2;
   Number const * d0 = a.tables + 2 * N2;
   Number const * d1 = d0 + N;
-  Number *       E = W;  // E[(s*4+c)*N2]: c = u-, du-/dxi_d, u+, du+/dxi_d
-  Number *       R = W;  // after F4, over E: R[(s*4+c)*N2]: c = value, t1, t2, d test coefficients
-  int const      t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
   int const      p = lt % N, q = lt / N;
-
-  int64_t nb[2];
-  int     type[2];
+  // E_j[(s*4+c)*N2]: c = u-, du-/dxi_d, u+, du+/dxi_d; after F4 the same storage holds
+  // R_j[(s*4+c)*N2]: c = value, t1, t2, d test coefficients
+  int type[ND][2];
 #pragma unroll
-  for(int s = 0; s < 2; ++s)
-  {
-    nb[s]   = neighbor_id(a.topo, cc, d, s);
-    type[s] = nb[s] >= 0 ? kInterior : a.topo.bc[2 * d + s];
-  }
+  for(int j = 0; j < ND; ++j)
+#pragma unroll
+    for(int s = 0; s < 2; ++s)
+    {
+      int64_t const nb = neighbor_id(a.topo, cc, dirs[j], s);
+      type[j][s]       = nb >= 0 ? kInterior : a.topo.bc[2 * dirs[j] + s];
+    }
 
   // F1: traces at the face nodes
   if(active)
-  {
-    int base, stride;
-    line_geometry<N>(d, p, q, base, stride);
-    Number u[N];
-    source_line<N, Number, mode>(a.src, cell, base, stride, a.probe, u);
-    E[0 * N2 + lt] = u[0];
-    E[1 * N2 + lt] = dot_line<N>(d0, u);
-    E[4 * N2 + lt] = u[N - 1];
-    E[5 * N2 + lt] = dot_line<N>(d1, u);
-    if constexpr(mode != ApplyMode::diagonal)
+#pragma unroll
+    for(int j = 0; j < ND; ++j)
     {
-      // neighbour traces, read at kernel start (their loads overlap the cell phases)
-      if(type[0] == kInterior)
+      int const d = dirs[j];
+      Number *  E = W + 8 * j * N2;
+      int       base, stride;
+      line_geometry<N>(d, p, q, base, stride);
+      Number u[N];
+      source_line<N, Number, mode>(a.src, cell, base, stride, a.probe, u);
+      E[0 * N2 + lt] = u[0];
+      E[1 * N2 + lt] = dot_line<N>(d0, u);
+      E[4 * N2 + lt] = u[N - 1];
+      E[5 * N2 + lt] = dot_line<N>(d1, u);
+      if constexpr(mode != ApplyMode::diagonal)
       {
-        E[2 * N2 + lt] = nbw[0];
-        E[3 * N2 + lt] = nbdw[0];
+        // neighbour traces, read at kernel start (their loads overlap the cell phases)
+        if(type[j][0] == kInterior)
+        {
+          E[2 * N2 + lt] = nbw[d][0];
+          E[3 * N2 + lt] = nbdw[d][0];
+        }
+        if(type[j][1] == kInterior)
+        {
+          E[6 * N2 + lt] = nbw[d][1];
+          E[7 * N2 + lt] = nbdw[d][1];
+        }
       }
-      if(type[1] == kInterior)
+      else
       {
-        E[6 * N2 + lt] = nbw[1];
-        E[7 * N2 + lt] = nbdw[1];
+        // probing the cell-local block: neighbours contribute zero, except a cell that is its
+        // own periodic neighbour (one cell along d), whose other end then acts as neighbour
+        bool const sl = neighbor_id(a.topo, cc, d, 0) == cell;
+        bool const sr = neighbor_id(a.topo, cc, d, 1) == cell;
+        E[2 * N2 + lt] = sl ? u[N - 1] : Number(0);
+        E[3 * N2 + lt] = sl ? dot_line<N>(d1, u) : Number(0);
+        E[6 * N2 + lt] = sr ? u[0] : Number(0);
+        E[7 * N2 + lt] = sr ? dot_line<N>(d0, u) : Number(0);
       }
     }
-    else
-    {
-      // probing the cell-local block: neighbours contribute zero, except a cell that is its
-      // own periodic neighbour (one cell along d), whose other end then acts as neighbour
-      bool const sl = nb[0] == cell, sr = nb[1] == cell;
-      E[2 * N2 + lt] = sl ? u[N - 1] : Number(0);
-      E[3 * N2 + lt] = sl ? dot_line<N>(d1, u) : Number(0);
-      E[6 * N2 + lt] = sr ? u[0] : Number(0);
-      E[7 * N2 + lt] = sr ? dot_line<N>(d0, u) : Number(0);
-    }
-  }
   __syncthreads();
-  // F2/F3: interpolate the 8 face arrays to the face Gauss points (S along t1, then t2)
+  // F2/F3: interpolate the 8 ND face arrays to the face Gauss points (S along t1, then t2)
   if(active)
-    for(int task = lt; task < 8 * N; task += N2)
+    for(int task = lt; task < 8 * ND * N; task += N2)
     {
-      Number * arr = E + (task / N) * N2;
-      int const r  = task % N;
+      Number *  arr = W + (task / N) * N2;
+      int const r   = task % N;
       Number    v[N], o[N];
       load_line<N>(arr + N * r, 1, v);
       mat_apply<N>(S, v, o);
@@ -259,103 +274,112 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
     }
   __syncthreads();
   if(active)
-    for(int task = lt; task < 8 * N; task += N2)
+    for(int task = lt; task < 8 * ND * N; task += N2)
     {
-      Number * arr = E + (task / N) * N2;
-      int const r  = task % N;
+      Number *  arr = W + (task / N) * N2;
+      int const r   = task % N;
       Number    v[N], o[N];
       load_line<N>(arr + r, N, v);
       mat_apply<N>(S, v, o);
       store_line<N>(arr + r, N, o);
     }
   __syncthreads();
-  // F4: fluxes at the 2 x N^2 face quadrature points (task s*N2 + lt: side s, point lt), kept
-  // in registers until every lane has read E, then written over it
-  Number res[2][4];
+  // F4: fluxes at the 2 x N^2 face quadrature points of each direction (side s, point lt),
+  // kept in registers until every lane has read E, then written over it
+  Number res[ND][2][4];
   if(active)
 #pragma unroll
-    for(int s = 0; s < 2; ++s)
-    {
-      int const qq = lt, q1 = qq % N, q2 = qq / N;
-      if(type[s] == EXADG_BC_NEUMANN)
-      {  // homogeneous Neumann: u+ = u-, du+/dn = -du-/dn -> no contribution
-        res[s][0] = res[s][1] = res[s][2] = res[s][3] = Number(0);
-        continue;
-      }
-      int64_t const  f  = face_id(a.topo, cc, d, s);
-      Number const * fd = a.face_data + f * 7 * N2;
-      Number         aK[3], aN[3];
+    for(int j = 0; j < ND; ++j)
 #pragma unroll
-      for(int j = 0; j < 3; ++j)
+      for(int s = 0; s < 2; ++s)
       {
-        Number const am = fd[j * N2 + qq], ap = fd[(3 + j) * N2 + qq];
-        aK[j]           = s ? am : -ap;
-        aN[j]           = s ? ap : -am;
-      }
-      Number const   w  = fd[6 * N2 + qq];
-      Number const * Es = E + s * 4 * N2;
-      Number         gt1 = 0, gt2 = 0;
+        int const d = dirs[j];
+        int const t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+        int const qq = lt, q1 = qq % N, q2 = qq / N;
+        if(type[j][s] == EXADG_BC_NEUMANN)
+        {  // homogeneous Neumann: u+ = u-, du+/dn = -du-/dn -> no contribution
+          res[j][s][0] = res[j][s][1] = res[j][s][2] = res[j][s][3] = Number(0);
+          continue;
+        }
+        int64_t const  f  = face_id(a.topo, cc, d, s);
+        Number const * fd = a.face_data + f * 7 * N2;
+        Number         aK[3], aN[3];
 #pragma unroll
-      for(int m = 0; m < N; ++m)
-      {
-        gt1 += Dq[q1 * N + m] * Es[m + N * q2];
-        gt2 += Dq[q2 * N + m] * Es[q1 + N * m];
-      }
-      Number const u      = Es[qq];
-      Number const dudn_K = aK[d] * Es[N2 + qq] + aK[t1] * gt1 + aK[t2] * gt2;
-      Number       uN, dudn_N;
-      if(type[s] == kInterior)
-      {
-        Number g1 = 0, g2 = 0;
+        for(int c = 0; c < 3; ++c)
+        {
+          Number const am = fd[c * N2 + qq], ap = fd[(3 + c) * N2 + qq];
+          aK[c]           = s ? am : -ap;
+          aN[c]           = s ? ap : -am;
+        }
+        Number const   w  = fd[6 * N2 + qq];
+        Number const * Es = W + 8 * j * N2 + s * 4 * N2;
+        Number         gt1 = 0, gt2 = 0;
 #pragma unroll
         for(int m = 0; m < N; ++m)
         {
-          g1 += Dq[q1 * N + m] * Es[2 * N2 + m + N * q2];
-          g2 += Dq[q2 * N + m] * Es[2 * N2 + q1 + N * m];
+          gt1 += Dq[q1 * N + m] * Es[m + N * q2];
+          gt2 += Dq[q2 * N + m] * Es[q1 + N * m];
         }
-        uN     = Es[2 * N2 + qq];
-        dudn_N = aN[d] * Es[3 * N2 + qq] + aN[t1] * g1 + aN[t2] * g2;
+        Number const u      = Es[qq];
+        Number const dudn_K = aK[d] * Es[N2 + qq] + aK[t1] * gt1 + aK[t2] * gt2;
+        Number       uN, dudn_N;
+        if(type[j][s] == kInterior)
+        {
+          Number g1 = 0, g2 = 0;
+#pragma unroll
+          for(int m = 0; m < N; ++m)
+          {
+            g1 += Dq[q1 * N + m] * Es[2 * N2 + m + N * q2];
+            g2 += Dq[q2 * N + m] * Es[2 * N2 + q1 + N * m];
+          }
+          uN     = Es[2 * N2 + qq];
+          dudn_N = aN[d] * Es[3 * N2 + qq] + aN[t1] * g1 + aN[t2] * g2;
+        }
+        else
+        {  // homogeneous Dirichlet: u+ = -u-, du+/dn = du-/dn
+          uN     = -u;
+          dudn_N = dudn_K;
+        }
+        Number const tau   = a.tau_face[f];
+        Number const jump  = u - uN;
+        Number const sigma = Number(-0.5) * jump * w;
+        res[j][s][0]       = -(Number(0.5) * (dudn_K + dudn_N) - tau * jump) * w;
+        res[j][s][1]       = sigma * aK[t1];
+        res[j][s][2]       = sigma * aK[t2];
+        res[j][s][3]       = sigma * aK[d];
       }
-      else
-      {  // homogeneous Dirichlet: u+ = -u-, du+/dn = du-/dn
-        uN     = -u;
-        dudn_N = dudn_K;
-      }
-      Number const tau   = a.tau_face[f];
-      Number const jump  = u - uN;
-      Number const sigma = Number(-0.5) * jump * w;
-      res[s][0]          = -(Number(0.5) * (dudn_K + dudn_N) - tau * jump) * w;
-      res[s][1]          = sigma * aK[t1];
-      res[s][2]          = sigma * aK[t2];
-      res[s][3]          = sigma * aK[d];
-    }
   __syncthreads();
   if(active)
 #pragma unroll
-    for(int s = 0; s < 2; ++s)
+    for(int j = 0; j < ND; ++j)
 #pragma unroll
-      for(int c = 0; c < 4; ++c)
-        R[(s * 4 + c) * N2 + lt] = res[s][c];
+      for(int s = 0; s < 2; ++s)
+#pragma unroll
+        for(int c = 0; c < 4; ++c)
+          W[8 * j * N2 + (s * 4 + c) * N2 + lt] = res[j][s][c];
   __syncthreads();
   // F5: tangential gradient test functions folded into the value test (Dq^T)
   if(active)
-    for(int task = lt; task < 2 * N2; task += N2)
-    {
-      int const s = task / N2, qq = task % N2, q1 = qq % N, q2 = qq / N;
-      Number *  Rs = R + s * 4 * N2;
-      Number    acc = Rs[qq];
 #pragma unroll
-      for(int m = 0; m < N; ++m)
-        acc += Dq[m * N + q1] * Rs[N2 + m + N * q2] + Dq[m * N + q2] * Rs[2 * N2 + q1 + N * m];
-      Rs[qq] = acc;
-    }
+    for(int j = 0; j < ND; ++j)
+#pragma unroll
+      for(int s = 0; s < 2; ++s)
+      {
+        int const qq = lt, q1 = qq % N, q2 = qq / N;
+        Number *  Rs = W + 8 * j * N2 + s * 4 * N2;
+        Number    acc = Rs[qq];
+#pragma unroll
+        for(int m = 0; m < N; ++m)
+          acc += Dq[m * N + q1] * Rs[N2 + m + N * q2] + Dq[m * N + q2] * Rs[2 * N2 + q1 + N * m];
+        Rs[qq] = acc;
+      }
   __syncthreads();
   // F6/F7: (S x S)^T back to the face nodes for the value and normal-derivative tests
   if(active)
-    for(int task = lt; task < 4 * N; task += N2)
+    for(int task = lt; task < 4 * ND * N; task += N2)
     {
-      int const which = task / N, r = task % N;
-      Number *  arr   = R + ((which >> 1) * 4 + ((which & 1) ? 3 : 0)) * N2;
+      int const which = task / N, r = task % N, j = which >> 2, w4 = which & 3;
+      Number *  arr   = W + 8 * j * N2 + ((w4 >> 1) * 4 + ((w4 & 1) ? 3 : 0)) * N2;
       Number    v[N], o[N];
       load_line<N>(arr + N * r, 1, v);
       mat_apply_T<N>(S, v, o);
@@ -363,39 +387,43 @@ __device__ __forceinline__ void general_face_direction(int d, SipgArgs<Number> c
     }
   __syncthreads();
   if(active)
-    for(int task = lt; task < 4 * N; task += N2)
+    for(int task = lt; task < 4 * ND * N; task += N2)
     {
-      int const which = task / N, r = task % N;
-      Number *  arr   = R + ((which >> 1) * 4 + ((which & 1) ? 3 : 0)) * N2;
+      int const which = task / N, r = task % N, j = which >> 2, w4 = which & 3;
+      Number *  arr   = W + 8 * j * N2 + ((w4 >> 1) * 4 + ((w4 & 1) ? 3 : 0)) * N2;
       Number    v[N], o[N];
       load_line<N>(arr + r, N, v);
       mat_apply_T<N>(S, v, o);
       store_line<N>(arr + r, N, o);
     }
   __syncthreads();
-  // F8: add to the own d-lines
-  if(active)
+  // F8: add to the own d-lines, one direction at a time (lines of two directions share DoFs)
+#pragma unroll
+  for(int j = 0; j < ND; ++j)
   {
-    int base, stride;
-    line_geometry<N>(d, p, q, base, stride);
-    Number v[N];
-    load_line<N>(A + base, stride, v);
-#pragma unroll
-    for(int s = 0; s < 2; ++s)
+    if(active)
     {
-      if(type[s] == EXADG_BC_NEUMANN)
-        continue;
-      Number const * Rs = R + s * 4 * N2;
-      Number const   rv = Rs[lt], rd = Rs[3 * N2 + lt];
-      Number const * dd = s ? d1 : d0;
+      int base, stride;
+      line_geometry<N>(dirs[j], p, q, base, stride);
+      Number v[N];
+      load_line<N>(A + base, stride, v);
 #pragma unroll
-      for(int i = 0; i < N; ++i)
-        v[i] += dd[i] * rd;
-      v[s ? N - 1 : 0] += rv;
+      for(int s = 0; s < 2; ++s)
+      {
+        if(type[j][s] == EXADG_BC_NEUMANN)
+          continue;
+        Number const * Rs = W + 8 * j * N2 + s * 4 * N2;
+        Number const   rv = Rs[lt], rd = Rs[3 * N2 + lt];
+        Number const * dd = s ? d1 : d0;
+#pragma unroll
+        for(int i = 0; i < N; ++i)
+          v[i] += dd[i] * rd;
+        v[s ? N - 1 : 0] += rv;
+      }
+      store_line<N>(A + base, stride, v);
     }
-    store_line<N>(A + base, stride, v);
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 // Minimum waves per SIMD asked of the compiler for the general kernel, matched to what the
@@ -415,7 +443,7 @@ template<int N, typename Number, ApplyMode mode>
 __global__ void __launch_bounds__(256)
   __attribute__((amdgpu_waves_per_eu(general_min_waves<N, Number>()))) sipg_general_kernel(SipgArgs<Number> a)
 {
-  constexpr int  N2 = N * N, N3 = N * N * N, WS = general_work_size<N>();
+  constexpr int  N2 = N * N, N3 = N * N * N, WS = general_work_size<N, Number>();
   constexpr int  CPB = cells_per_block(N);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Number *       smem = reinterpret_cast<Number *>(smem_raw);
@@ -580,9 +608,21 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
+  if constexpr(face_dirs_per_round<Number>() == 2)
+  {
+    int const xy[2] = {0, 1}, z[1] = {2};
+    general_faces<N, Number, mode, 2>(xy, a, cell, cc, active, lt, A, W, nbw, nbdw);
+    general_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw);
+  }
+  else
+  {
 #pragma unroll
-  for(int d = 0; d < 3; ++d)
-    general_face_direction<N, Number, mode>(d, a, cell, cc, active, lt, A, W, nbw[d], nbdw[d]);
+    for(int d = 0; d < 3; ++d)
+    {
+      int const one[1] = {d};
+      general_faces<N, Number, mode, 1>(one, a, cell, cc, active, lt, A, W, nbw, nbdw);
+    }
+  }
 
   if(active)
   {
@@ -933,7 +973,7 @@ hipError_t launch_general_n(SipgArgs<Number> const & a, hipStream_t stream)
 {
   constexpr int CPB    = cells_per_block(N);
   int const     blocks = (int)((a.n_cells + CPB - 1) / CPB);
-  size_t const  lds    = sizeof(Number) * CPB * (N * N * N + general_work_size<N>());
+  size_t const  lds    = sizeof(Number) * CPB * (N * N * N + general_work_size<N, Number>());
   static bool   attr   = false;
   if(!attr && lds > 65536)
   {
