@@ -57,8 +57,9 @@ def curved_bytes(k, n_cells_1d, s=8):
 
 
 def run_sweep(ex, torch, args):
-    """Config 2, curved part: [0,1]^3 deformed by x_d += 0.1 prod sin(pi x_e), MappingQ(k),
-    FP64 vmult, seeded uniform[-1,1] input, median of HIP-event timings."""
+    """Curved-mesh operator sweep: [0,1]^3 deformed by x_d += 0.1 prod sin(pi x_e), MappingQ(k),
+    k = 2, 3, 5, 7 at ~23.9M DoF each, FP64 vmult, seeded uniform[-1,1] input, median of
+    HIP-event timings."""
     res = []
     for k, m in ((2, 96), (3, 72), (5, 48), (7, 36)):
         op = ex.LaplaceOperator(ex.mesh_desc(m, deformation=0.1, frequency=1, mapping_degree=k),
@@ -88,7 +89,8 @@ def run_sweep(ex, torch, args):
 
 
 def run_guess(ex, torch, args):
-    """Config 4: pressure sequence on a periodic-x/y walled-z box; train on the first steps,
+    """Pressure sequence on a periodic-x/y walled-z box (32^3 cells, k=4, 4.1M DoF by default);
+    train on the first steps,
     A/B on the rest. FP32-MG arms: zero / extrapolated / learned guess; FP64-MG arms (the plain
     method): zero / extrapolated guess."""
     from exadg_amd import pressure_sequence as ps
@@ -135,8 +137,8 @@ def run_guess(ex, torch, args):
 
 
 def run_combined(ex, torch, args):
-    """Config 5: the pressure sequence of config 4 on the curved mesh of config 3 (k=5,
-    deformation 0.1, MappingQ(5)), periodic x/y, Neumann walls z. Train on the first steps,
+    """The pressure sequence on a curved mesh (50^3 cells, k=5, deformation 0.1, MappingQ(5),
+    27M DoF), periodic x/y, Neumann walls z. Train on the first steps,
     then A/B on the rest: FP32 V-cycle with the learned / extrapolated guess against the plain
     FP64 method (FP64 V-cycle, extrapolated or zero guess)."""
     from exadg_amd import pressure_sequence as ps
@@ -193,11 +195,11 @@ def main():
     ap.add_argument("--degree", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true",
-                    help="also time the CG + p-multigrid solve (config 1 and curved k=5 48^3)")
+                    help="also time the CG + p-multigrid solve (Cartesian k=3 8^3 and curved k=5 48^3)")
     ap.add_argument("--sweep", action="store_true",
                     help="also time the curved-mesh vmult at degrees 2, 3, 5, 7 (~24M DoF each)")
     ap.add_argument("--guess", action="store_true",
-                    help="also run the learned-initial-guess pressure sequence (config 4)")
+                    help="also run the learned-initial-guess pressure sequence (32^3, k=4)")
     ap.add_argument("--guess-cells", type=int, default=32)
     ap.add_argument("--guess-degree", type=int, default=4)
     ap.add_argument("--guess-steps", type=int, default=400)
@@ -206,7 +208,7 @@ def main():
     ap.add_argument("--guess-kmax", type=int, default=8)
     ap.add_argument("--guess-lr", type=float, default=1e-3)
     ap.add_argument("--combined", action="store_true",
-                    help="also run the combined experiment (config 5): curved k=5 pressure sequence")
+                    help="also run the combined experiment: curved k=5 pressure sequence (50^3)")
     ap.add_argument("--combined-cells", type=int, default=50)
     ap.add_argument("--combined-steps", type=int, default=200)
     ap.add_argument("--combined-train", type=int, default=150)
@@ -287,10 +289,10 @@ def main():
         x = y = None
         torch.cuda.empty_cache()
         sol = {}
-        sol["config1_cart_k3_8c"] = solve_bench.compare_arms(
+        sol["cart_k3_8c"] = solve_bench.compare_arms(
             exadg_amd, torch, exadg_amd.mesh_desc(8), exadg_amd.bc_desc(), 3, [3, 1], 3.141592653589793,
             exadg_amd.COARSE_CG, reps=3)
-        sol["config3_curved_k5_48c"] = solve_bench.compare_arms(
+        sol["curved_k5_48c"] = solve_bench.compare_arms(
             exadg_amd, torch, exadg_amd.mesh_desc(48, deformation=0.1, mapping_degree=5),
             exadg_amd.bc_desc(), 5, [5, 3, 2, 1], 3.141592653589793, exadg_amd.COARSE_CHEBYSHEV)
         out["solve"] = sol

@@ -1,8 +1,8 @@
-// Kernels of the learned initial guess (per-cell MLP, FP32): a tiled SGEMM with strided
-// operands, feature gathering from a solution history, ReLU and MSE backward, bias
-// gradients, Adam; plus the FP64 extrapolation of the initial guess
-// (time_integration/extrapolation_constants.cpp:55-93) and the config-4 right-hand side:
-// the weak DG divergence with central flux (divergence_operator.h:63-68,
+// Kernels of the learned initial guess (per-cell MLP, FP32): a tiled GEMM on the matrix cores
+// with strided operands, feature gathering from a solution history, ReLU and MSE backward,
+// bias gradients, Adam; plus the FP64 extrapolation of the initial guess
+// (time_integration/extrapolation_constants.cpp:55-93) and the right-hand side of the
+// pressure sequence: the weak DG divergence with central flux (divergence_operator.h:63-68,
 // divergence_operator.cpp:190-215) of an analytic Fourier velocity field.
 #include <hip/hip_runtime.h>
 

@@ -190,7 +190,7 @@ def test_pcg_jacobi_matches_oracle_iterations(torch_cuda, product):
     assert info2["converged"]
 
 
-def test_pcg_config1_dirichlet_box(torch_cuda, product):
+def test_pcg_cartesian_k3_8c_dirichlet_box(torch_cuda, product):
     """[0,1]^3, 8^3 cells, k=3, u = prod sin(pi x), Dirichlet; P = DG3 -> DG1 -> CG-Q1, FP32 MG."""
     torch = torch_cuda
     op = product.LaplaceOperator(product.mesh_desc(8), product.bc_desc(), 3)

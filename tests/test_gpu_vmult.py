@@ -1,6 +1,6 @@
 """GPU parity of the SIPG vmult, diagonal and RHS against the CPU oracle (through the C ABI).
 
-Bar: FP64 relative error <= 1e-12 (north star); FP32 <= 2e-6 relative to ||A x||.
+Bar: FP64 relative error <= 1e-12; FP32 <= 2e-6 relative to ||A x||.
 """
 import numpy as np
 import pytest
