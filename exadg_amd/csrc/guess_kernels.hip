@@ -6,6 +6,8 @@
 // divergence_operator.cpp:190-215) of an analytic Fourier velocity field.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "guess_kernels.h"
 
 namespace ExaDG
@@ -779,14 +781,20 @@ hipError_t launch_divergence_rhs_curved(double * b, double const * mode_data, in
   if(n < 1 || n > 9 || K < 0 || K > 16 || n_cells < 1)
     return hipErrorInvalidValue;
   size_t const lds = sizeof(double) * (5 * n * n * n + 128 * 6 * (K + 1));
-  static size_t attr = 0;
-  if(lds > 65536 && lds > attr)
+  // raised under a lock to the largest size asked for so far, so it never decreases
+  static std::mutex attr_lock;
+  static size_t     attr = 0;
+  if(lds > 65536)
   {
-    hipError_t e = hipFuncSetAttribute((void const *)divergence_rhs_curved_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if(e != hipSuccess)
-      return e;
-    attr = lds;
+    std::lock_guard<std::mutex> guard(attr_lock);
+    if(lds > attr)
+    {
+      hipError_t e = hipFuncSetAttribute((void const *)divergence_rhs_curved_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if(e != hipSuccess)
+        return e;
+      attr = lds;
+    }
   }
   hipLaunchKernelGGL(divergence_rhs_curved_kernel, dim3(n_cells), dim3(128), lds, s, b, mode_data,
                      n_modes, K, topo, n, S, Sd, w, cell_geo, face_geo);

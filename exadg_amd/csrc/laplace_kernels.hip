@@ -11,6 +11,8 @@
 // homogeneous mirror BCs of weak_boundary_conditions.h:39-47,142-160.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "../../include/exadg_amd.h"
 #include "laplace_kernels.h"
 
@@ -974,14 +976,15 @@ hipError_t launch_general_n(SipgArgs<Number> const & a, hipStream_t stream)
   constexpr int CPB    = cells_per_block(N);
   int const     blocks = (int)((a.n_cells + CPB - 1) / CPB);
   size_t const  lds    = sizeof(Number) * CPB * (N * N * N + general_work_size<N, Number>());
-  static bool   attr   = false;
-  if(!attr && lds > 65536)
+  // set once per instantiation; concurrent first calls may both set it, which is harmless
+  static std::atomic<bool> attr{false};
+  if(lds > 65536 && !attr.load(std::memory_order_acquire))
   {
     hipError_t e = hipFuncSetAttribute((void const *)sipg_general_kernel<N, Number, mode>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if(e != hipSuccess)
       return e;
-    attr = true;
+    attr.store(true, std::memory_order_release);
   }
   hipLaunchKernelGGL((sipg_general_kernel<N, Number, mode>), dim3(blocks), dim3(256), lds, stream, a);
   return hipGetLastError();
