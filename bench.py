@@ -228,9 +228,8 @@ def main():
     mesh = exadg_amd.mesh_desc(args.cells)
     op = exadg_amd.LaplaceOperator(mesh, exadg_amd.bc_desc(), args.degree, exadg_amd.FP64)
     n = op.n
-    # seeded uniform[-1,1] input, generated on the host once and moved to HBM
-    import oracle
-    x = torch.from_numpy(oracle.random_vector(n)).cuda()
+    # seeded uniform[-1,1] input (SplitMix64 on the DoF index), generated in HBM
+    x = exadg_amd.random_vector(n)
     y = torch.empty_like(x)
     for _ in range(args.warmup):
         op.vmult(y, x)

@@ -154,6 +154,7 @@ def lib():
             "exadg_fourier_modes": [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ip],
             "exadg_laplace_rhs_divergence_fourier": [vp, dp, ctypes.c_int, ctypes.c_double, vp, vp],
             "exadg_vector_subtract_mean": [vp, ctypes.c_int64, vp],
+            "exadg_vector_random": [vp, ctypes.c_int64, ctypes.c_int, ctypes.c_uint64, vp],
             "exadg_extrapolate": [ctypes.c_int, ctypes.POINTER(vp), vp, ctypes.c_int64, vp],
             "exadg_guess_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                    ctypes.POINTER(vp)],
@@ -353,6 +354,19 @@ def fourier_modes(seed, k_min=1, k_max=8):
                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                     ctypes.byref(cnt)))
     return out
+
+
+DEFAULT_SEED = 0x5EED2026
+
+
+def random_vector(n, seed=DEFAULT_SEED, precision=None, stream=None):
+    """Device vector of n uniform[-1,1] values, SplitMix64 on (seed, index) (exadg_vector_random);
+    FP64 unless precision is FP32."""
+    import torch
+    precision = FP64 if precision is None else precision
+    v = torch.empty(n, dtype=torch.float64 if precision == FP64 else torch.float32, device="cuda")
+    check(lib().exadg_vector_random(_ptr(v), n, precision, seed, _stream(stream)))
+    return v
 
 
 def subtract_mean(v, stream=None):

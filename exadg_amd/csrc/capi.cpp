@@ -394,6 +394,19 @@ int exadg_laplace_rhs_divergence_fourier(exadg_laplace * op, const double * mode
   });
 }
 
+int exadg_vector_random(void * v, int64_t n, int precision, uint64_t seed, void * stream)
+{
+  return guarded([&] {
+    require(v != nullptr && n >= 0, "null vector or negative length");
+    if(precision == EXADG_FP64)
+      ExaDG::vec::random_uniform<double>(static_cast<double *>(v), n, seed, stream_of(stream));
+    else if(precision == EXADG_FP32)
+      ExaDG::vec::random_uniform<float>(static_cast<float *>(v), n, seed, stream_of(stream));
+    else
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "precision must be EXADG_FP64 or EXADG_FP32");
+  });
+}
+
 int exadg_vector_subtract_mean(double * v, int64_t n, void * stream)
 {
   return guarded([&] {

@@ -82,6 +82,22 @@ __global__ void __launch_bounds__(kBlock) set_zero_kernel(Number * x, int64_t n)
     x[i] = Number(0);
 }
 
+// x[i] = uniform[-1,1] from the SplitMix64 mixer applied to (seed, i): a counter-based
+// generator, so any device, grid or host loop produces the same vector
+template<typename Number>
+__global__ void __launch_bounds__(kBlock) random_uniform_kernel(Number * x, int64_t n, uint64_t seed)
+{
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+  {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+    z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z          = z ^ (z >> 31);
+    x[i]       = (Number)(2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
+  }
+}
+
 template<typename To, typename From>
 __global__ void __launch_bounds__(kBlock) convert_kernel(To * dst, From const * src, int64_t n)
 {
@@ -214,6 +230,14 @@ void set_zero(Number * x, int64_t n, hipStream_t s)
 }
 
 template<typename Number>
+void random_uniform(Number * x, int64_t n, uint64_t seed, hipStream_t s)
+{
+  hipLaunchKernelGGL((random_uniform_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, x, n,
+                     seed);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number>
 void copy(Number * dst, Number const * src, int64_t n, hipStream_t s)
 {
   HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(Number), hipMemcpyDeviceToDevice, s));
@@ -308,6 +332,7 @@ void subtract_mean(Reducer & red, Number * x, int64_t n, hipStream_t s)
 
 #define EXADG_INST(T)                                                                            \
   template void set_zero<T>(T *, int64_t, hipStream_t);                                          \
+  template void random_uniform<T>(T *, int64_t, uint64_t, hipStream_t);                          \
   template void copy<T>(T *, T const *, int64_t, hipStream_t);                                   \
   template void dot<T>(Reducer &, int, T const *, T const *, int64_t, hipStream_t);              \
   template void norm2_and_dot<T>(Reducer &, int, int, T const *, T const *, int64_t, hipStream_t); \

@@ -54,6 +54,9 @@ namespace vec
 {
 template<typename Number>
 void set_zero(Number * x, int64_t n, hipStream_t s);
+// x[i] = uniform[-1,1] by SplitMix64 on (seed, i), reproducible on any grid
+template<typename Number>
+void random_uniform(Number * x, int64_t n, uint64_t seed, hipStream_t s);
 template<typename Number>
 void copy(Number * dst, Number const * src, int64_t n, hipStream_t s);
 template<typename To, typename From>

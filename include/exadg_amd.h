@@ -229,6 +229,12 @@ int exadg_fourier_modes(uint64_t seed, int k_min, int k_max, int max_modes, doub
 int exadg_laplace_rhs_divergence_fourier(exadg_laplace * op, const double * modes, int n_modes,
                                          double t, double * b, void * stream);
 
+/* v[i] = uniform[-1,1] from SplitMix64 on (seed, i) (device, FP64 or FP32 by `precision`): the
+ * source vector of the operator-throughput measurement. The reference driver applies the
+ * operator to src = 1 (Driver::apply_operator, poisson/driver.cpp:190-199); a seeded
+ * counter-based vector gives the same input on the host and the device at any size. */
+int exadg_vector_random(void * v, int64_t n, int precision, uint64_t seed, void * stream);
+
 /* v -= mean(v) (FP64, device): the pressure level of a singular problem. */
 int exadg_vector_subtract_mean(double * v, int64_t n, void * stream);
 

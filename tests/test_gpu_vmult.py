@@ -172,3 +172,15 @@ def test_error_sine_vs_oracle(torch_cuda, product, k, curved):
     e_gpu = op.error_sine(1.3 * np.pi, torch.from_numpy(u).cuda())
     e_ref = ref.error_sine(1.3 * np.pi, u)
     assert abs(e_gpu - e_ref) <= 1e-12 * e_ref
+
+
+def test_random_vector_matches_oracle(torch_cuda, product):
+    """exadg_vector_random is the counter-based SplitMix64 vector of the oracle, bit for bit, at
+    any length (the grid-stride loop covers lengths beyond one grid), in FP64 and FP32."""
+    for n, seed in ((1, 7), (1000, oracle.DEFAULT_SEED), (3_000_017, 12345)):
+        v = product.random_vector(n, seed=seed)
+        assert np.array_equal(v.cpu().numpy(), oracle.random_vector(n, seed=seed))
+    v32 = product.random_vector(1000, precision=product.FP32)
+    assert np.array_equal(v32.cpu().numpy(), oracle.random_vector(1000).astype(np.float32))
+    assert not np.array_equal(product.random_vector(64, seed=1).cpu().numpy(),
+                              product.random_vector(64, seed=2).cpu().numpy())
