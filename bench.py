@@ -186,6 +186,33 @@ def run_combined(ex, torch, args):
     }
 
 
+def measure_stream(torch, x, y, reps=20):
+    """The box's practical HBM bandwidth on vectors of the benchmark's size, next to the vendor
+    peak: a device-to-device copy (2 vectors moved) and a triad a = b + s c (3 vectors),
+    medians of HIP-event timings."""
+    z = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+
+    def median_ms(f):
+        f()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for a, b in ev:
+            a.record(stream)
+            f()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return statistics.median(a.elapsed_time(b) for a, b in ev)
+
+    nbytes = x.numel() * x.element_size()
+    t_copy = median_ms(lambda: y.copy_(x))
+    t_triad = median_ms(lambda: torch.add(x, y, alpha=0.5, out=z))
+    del z
+    return {"copy_GBps": round(2 * nbytes / t_copy / 1e6, 1),
+            "triad_GBps": round(3 * nbytes / t_triad / 1e6, 1),
+            "bytes_per_vector": nbytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,6 +277,7 @@ def main():
         for _ in range(args.steps):
             y.copy_(x)
         torch.cuda.synchronize()
+    stream_bw = measure_stream(torch, x, y)
     ms_step = wall * 1e3 / args.steps
     t_med = statistics.median(kernel_ms) * 1e-3
     ab = algorithmic_bytes(n, op.n_cells)
@@ -282,6 +310,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
+        "stream": stream_bw,
     }
     if args.solve:
         from exadg_amd import solve_bench
