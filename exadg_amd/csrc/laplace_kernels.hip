@@ -190,7 +190,9 @@ constexpr int general_work_size()
 {
   // cell phase: the x and y gradients (the z one stays in registers); face phases: 8 trace
   // arrays per direction of a round, which the flux coefficients then overwrite
-  constexpr int faces = 8 * face_dirs_per_round<Number>() * N * N;
+  // (one direction per round: 12 n^2, the flux results written at once, see general_faces)
+  constexpr int faces = face_dirs_per_round<Number>() == 1 ? 12 * N * N
+                                                           : 8 * face_dirs_per_round<Number>() * N * N;
   return (2 * N * N * N > faces) ? 2 * N * N * N : faces;
 }
 
@@ -209,8 +211,16 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
   Number const * d0 = a.tables + 2 * N2;
   Number const * d1 = d0 + N;
   int const      p = lt % N, q = lt / N;
-  // E_j[(s*4+c)*N2]: c = u-, du-/dxi_d, u+, du+/dxi_d; after F4 the same storage holds
-  // R_j[(s*4+c)*N2]: c = value, t1, t2, d test coefficients
+  // E_j[(s*4+c)*N2]: c = u-, du-/dxi_d, u+, du+/dxi_d. F4 reads u- and u+ across lanes but
+  // the derivatives only at the lane's own point, and produces four arrays per side: value,
+  // t1, t2 and d test coefficients. With one direction per round they are written at once, t1
+  // and d over the own-point derivative slots, value and t2 into 4 N^2 more; with two, the
+  // results wait in registers for a barrier and then overwrite E_j.
+  constexpr bool direct = ND == 1;
+  auto Rv  = [&](int j, int s) { return direct ? W + 8 * N2 + s * 2 * N2 : W + (8 * j + s * 4) * N2; };
+  auto Rt1 = [&](int j, int s) { return direct ? W + (s * 4 + 1) * N2 : W + (8 * j + s * 4 + 1) * N2; };
+  auto Rt2 = [&](int j, int s) { return direct ? W + 9 * N2 + s * 2 * N2 : W + (8 * j + s * 4 + 2) * N2; };
+  auto Rd  = [&](int j, int s) { return direct ? W + (s * 4 + 3) * N2 : W + (8 * j + s * 4 + 3) * N2; };
   int type[ND][2];
 #pragma unroll
   for(int j = 0; j < ND; ++j)
@@ -301,6 +311,8 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
         if(type[j][s] == EXADG_BC_NEUMANN)
         {  // homogeneous Neumann: u+ = u-, du+/dn = -du-/dn -> no contribution
           res[j][s][0] = res[j][s][1] = res[j][s][2] = res[j][s][3] = Number(0);
+          if constexpr(direct)
+            Rv(j, s)[qq] = Rt1(j, s)[qq] = Rt2(j, s)[qq] = Rd(j, s)[qq] = Number(0);
           continue;
         }
         int64_t const  f  = face_id(a.topo, cc, d, s);
@@ -349,17 +361,30 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
         res[j][s][1]       = sigma * aK[t1];
         res[j][s][2]       = sigma * aK[t2];
         res[j][s][3]       = sigma * aK[d];
+        if constexpr(direct)
+        {
+          Rv(j, s)[qq]  = res[j][s][0];
+          Rt1(j, s)[qq] = res[j][s][1];
+          Rt2(j, s)[qq] = res[j][s][2];
+          Rd(j, s)[qq]  = res[j][s][3];
+        }
       }
   __syncthreads();
-  if(active)
+  if constexpr(!direct)
+  {
+    if(active)
 #pragma unroll
-    for(int j = 0; j < ND; ++j)
+      for(int j = 0; j < ND; ++j)
 #pragma unroll
-      for(int s = 0; s < 2; ++s)
-#pragma unroll
-        for(int c = 0; c < 4; ++c)
-          W[8 * j * N2 + (s * 4 + c) * N2 + lt] = res[j][s][c];
-  __syncthreads();
+        for(int s = 0; s < 2; ++s)
+        {
+          Rv(j, s)[lt]  = res[j][s][0];
+          Rt1(j, s)[lt] = res[j][s][1];
+          Rt2(j, s)[lt] = res[j][s][2];
+          Rd(j, s)[lt]  = res[j][s][3];
+        }
+    __syncthreads();
+  }
   // F5: tangential gradient test functions folded into the value test (Dq^T)
   if(active)
 #pragma unroll
@@ -367,13 +392,15 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
 #pragma unroll
       for(int s = 0; s < 2; ++s)
       {
-        int const qq = lt, q1 = qq % N, q2 = qq / N;
-        Number *  Rs = W + 8 * j * N2 + s * 4 * N2;
-        Number    acc = Rs[qq];
+        int const      qq = lt, q1 = qq % N, q2 = qq / N;
+        Number *       rv = Rv(j, s);
+        Number const * r1 = Rt1(j, s);
+        Number const * r2 = Rt2(j, s);
+        Number         acc = rv[qq];
 #pragma unroll
         for(int m = 0; m < N; ++m)
-          acc += Dq[m * N + q1] * Rs[N2 + m + N * q2] + Dq[m * N + q2] * Rs[2 * N2 + q1 + N * m];
-        Rs[qq] = acc;
+          acc += Dq[m * N + q1] * r1[m + N * q2] + Dq[m * N + q2] * r2[q1 + N * m];
+        rv[qq] = acc;
       }
   __syncthreads();
   // F6/F7: (S x S)^T back to the face nodes for the value and normal-derivative tests
@@ -381,7 +408,7 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
     for(int task = lt; task < 4 * ND * N; task += N2)
     {
       int const which = task / N, r = task % N, j = which >> 2, w4 = which & 3;
-      Number *  arr   = W + 8 * j * N2 + ((w4 >> 1) * 4 + ((w4 & 1) ? 3 : 0)) * N2;
+      Number *  arr   = (w4 & 1) ? Rd(j, w4 >> 1) : Rv(j, w4 >> 1);
       Number    v[N], o[N];
       load_line<N>(arr + N * r, 1, v);
       mat_apply_T<N>(S, v, o);
@@ -392,7 +419,7 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
     for(int task = lt; task < 4 * ND * N; task += N2)
     {
       int const which = task / N, r = task % N, j = which >> 2, w4 = which & 3;
-      Number *  arr   = W + 8 * j * N2 + ((w4 >> 1) * 4 + ((w4 & 1) ? 3 : 0)) * N2;
+      Number *  arr   = (w4 & 1) ? Rd(j, w4 >> 1) : Rv(j, w4 >> 1);
       Number    v[N], o[N];
       load_line<N>(arr + r, N, v);
       mat_apply_T<N>(S, v, o);
@@ -414,8 +441,7 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
       {
         if(type[j][s] == EXADG_BC_NEUMANN)
           continue;
-        Number const * Rs = W + 8 * j * N2 + s * 4 * N2;
-        Number const   rv = Rs[lt], rd = Rs[3 * N2 + lt];
+        Number const   rv = Rv(j, s)[lt], rd = Rd(j, s)[lt];
         Number const * dd = s ? d1 : d0;
 #pragma unroll
         for(int i = 0; i < N; ++i)
