@@ -397,7 +397,12 @@ int exadg_laplace_rhs_divergence_fourier(exadg_laplace * op, const double * mode
 int exadg_vector_random(void * v, int64_t n, int precision, uint64_t seed, void * stream)
 {
   return guarded([&] {
-    require(v != nullptr && n >= 0, "null vector or negative length");
+    require(n >= 0, "negative length");
+    if(precision != EXADG_FP64 && precision != EXADG_FP32)
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "precision must be EXADG_FP64 or EXADG_FP32");
+    if(n == 0)
+      return;
+    require(v != nullptr, "null vector");
     if(precision == EXADG_FP64)
       ExaDG::vec::random_uniform<double>(static_cast<double *>(v), n, seed, stream_of(stream));
     else if(precision == EXADG_FP32)

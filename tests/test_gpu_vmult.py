@@ -184,3 +184,4 @@ def test_random_vector_matches_oracle(torch_cuda, product):
     assert np.array_equal(v32.cpu().numpy(), oracle.random_vector(1000).astype(np.float32))
     assert not np.array_equal(product.random_vector(64, seed=1).cpu().numpy(),
                               product.random_vector(64, seed=2).cpu().numpy())
+    assert product.random_vector(0).numel() == 0
