@@ -67,7 +67,8 @@ class SolverData(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
                 ("not_finite", ctypes.c_int32), ("r0", ctypes.c_double), ("r_final", ctypes.c_double),
-                ("b_norm", ctypes.c_double), ("rho", ctypes.c_double), ("n10", ctypes.c_double)]
+                ("b_norm", ctypes.c_double), ("rho", ctypes.c_double), ("n10", ctypes.c_double),
+                ("r_true", ctypes.c_double), ("tolerance", ctypes.c_double)]
 
 
 PRECOND_NONE, PRECOND_POINT_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
@@ -189,6 +190,36 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _require(cond, msg):
+    if not cond:
+        raise ValueError(msg)
+
+
+def check_vector(t, n, dtype, what="vector"):
+    """The C entry points take bare device pointers, so every wrapper checks a tensor before it
+    is passed: n elements of `dtype`, contiguous, on the GPU. Raises ValueError otherwise (before
+    any kernel launch)."""
+    _require(t is not None, f"{what}: missing")
+    _require(t.numel() == n, f"{what}: {t.numel()} entries, expected {n}")
+    _require(t.dtype == dtype, f"{what}: dtype {t.dtype}, expected {dtype}")
+    _require(t.is_contiguous(), f"{what}: must be contiguous")
+    _require(t.is_cuda, f"{what}: must be a device tensor")
+
+
+def check_sequence(P, Bv, t_begin, t_end, history=0):
+    """A stored solution/right-hand-side sequence (steps x n_dofs, FP64, device) and the step
+    range [t_begin, t_end) that a GuessNet call reads (history steps before t_begin + history
+    included): both tensors 2-D of the same shape, 0 <= t_begin < t_end <= steps."""
+    import torch
+    _require(P.dim() == 2 and Bv.dim() == 2, "P and Bv must be 2-D (steps x n_dofs)")
+    _require(tuple(P.shape) == tuple(Bv.shape), "P and Bv must have the same shape")
+    _require(0 <= t_begin < t_end <= P.shape[0],
+             f"step range [{t_begin}, {t_end}) outside the {P.shape[0]} stored steps")
+    _require(t_end - t_begin > history, "step range shorter than the history")
+    for t, what in ((P, "P"), (Bv, "Bv")):
+        check_vector(t, t.numel(), torch.float64, what)
+
+
 def _stream(stream):
     if stream is None:
         import torch
@@ -232,8 +263,8 @@ class LaplaceOperator:
         import torch
         return torch.float64 if self.precision == FP64 else torch.float32
 
-    def _check_vec(self, t):
-        assert t.is_cuda and t.dtype == self.dtype and t.numel() == self.n and t.is_contiguous()
+    def _check_vec(self, t, what="vector"):
+        check_vector(t, self.n, self.dtype, what)
 
     def vmult(self, dst, src, stream=None):
         self._check_vec(dst)
@@ -255,13 +286,13 @@ class LaplaceOperator:
 
     def rhs_sine(self, omega, b, stream=None):
         import torch
-        assert b.is_cuda and b.dtype == torch.float64 and b.numel() == self.n
+        check_vector(b, self.n, torch.float64, "b")
         check(lib().exadg_laplace_rhs_sine(self._h, omega, _ptr(b), _stream(stream)))
 
     def error_sine(self, omega, u, stream=None):
         """Relative L2 error against prod sin(omega x_d), QGauss(k+3) (FP64 operator)."""
         import torch
-        assert u.is_cuda and u.dtype == torch.float64 and u.numel() == self.n
+        check_vector(u, self.n, torch.float64, "u")
         e = ctypes.c_double()
         check(lib().exadg_laplace_error_sine(self._h, omega, _ptr(u), ctypes.byref(e),
                                              _stream(stream)))
@@ -272,9 +303,9 @@ class LaplaceOperator:
         (numpy float64, 10 per mode) at time t; mean removed for a singular operator."""
         import numpy as np
         import torch
-        assert b.is_cuda and b.dtype == torch.float64 and b.numel() == self.n
+        check_vector(b, self.n, torch.float64, "b")
         m = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1)
-        assert m.size % 10 == 0 and m.size > 0
+        _require(m.size % 10 == 0 and m.size > 0, "modes: 10 doubles per mode expected")
         check(lib().exadg_laplace_rhs_divergence_fourier(
             self._h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m.size // 10, t, _ptr(b),
             _stream(stream)))
@@ -285,6 +316,10 @@ class Preconditioner:
 
     def __init__(self, op, kind, mg=None, stream=None):
         self.kind = kind
+        # the C preconditioner refers to the operator (point Jacobi) and was built for its size:
+        # keep the operator alive as long as this object
+        self._op = op
+        self.n = op.n
         self._h = ctypes.c_void_p()
         check(lib().exadg_preconditioner_create(op.handle, kind,
                                                 ctypes.byref(mg) if mg is not None else None,
@@ -306,6 +341,9 @@ class Preconditioner:
             pass
 
     def vmult(self, dst, src, stream=None):
+        import torch
+        check_vector(dst, self.n, torch.float64, "dst")
+        check_vector(src, self.n, torch.float64, "src")
         check(lib().exadg_preconditioner_vmult(self._h, _ptr(dst), _ptr(src), _stream(stream)))
 
     def update(self, stream=None):
@@ -313,6 +351,10 @@ class Preconditioner:
 
     def smooth(self, op, b, x, stream=None):
         """x <- x + S (b - A x), S the finest-level smoother (multigrid only)."""
+        import torch
+        _require(op.n == self.n, "operator and preconditioner sizes differ")
+        check_vector(b, self.n, torch.float64, "b")
+        check_vector(x, self.n, torch.float64, "x")
         check(lib().exadg_preconditioner_smooth(op.handle, self._h, _ptr(b), _ptr(x), _stream(stream)))
 
     def mg_info(self):
@@ -330,14 +372,20 @@ class Preconditioner:
 
 def cg_solve(op, b, x, prec=None, rel_tol=1e-10, abs_tol=1e-20, max_iter=10000, rel_mode=0,
              stream=None, raise_on_error=True):
-    """Krylov::KrylovSolver::solve (CG) on the FP64 operator; x holds the initial guess."""
+    """Krylov::KrylovSolver::solve (CG) on the FP64 operator; x holds the initial guess.
+    The result carries the final FP64 true residual r_true = ||b - A x||."""
+    import torch
+    _require(prec is None or prec.n == op.n, "operator and preconditioner sizes differ")
+    check_vector(b, op.n, torch.float64, "b")
+    check_vector(x, op.n, torch.float64, "x")
     d = SolverData(max_iter, abs_tol, rel_tol, rel_mode)
     info = SolveInfo()
     rc = lib().exadg_cg_solve(op.handle, prec.handle if prec is not None else None, _ptr(b), _ptr(x),
                               ctypes.byref(d), ctypes.byref(info), _stream(stream))
     out = dict(iterations=info.iterations, converged=bool(info.converged),
                not_finite=bool(info.not_finite), r0=info.r0, r_final=info.r_final,
-               b_norm=info.b_norm, rho=info.rho, n10=info.n10, status=rc)
+               b_norm=info.b_norm, rho=info.rho, n10=info.n10, r_true=info.r_true,
+               tolerance=info.tolerance, status=rc)
     if rc != EXADG_OK and raise_on_error:
         raise ExaDGError(rc, lib().exadg_last_error().decode())
     return out
@@ -384,8 +432,11 @@ def _history(vectors):
 
 def extrapolate(history, x0, stream=None):
     """x0 = sum_i beta_i history[i] (history newest first, FP64 device vectors)."""
+    import torch
+    _require(1 <= len(history) <= 4, "extrapolation order must be in [1,4]")
     for t in history:
-        assert t.is_cuda and t.numel() == x0.numel() and t.dtype == x0.dtype
+        check_vector(t, x0.numel(), torch.float64, "history vector")
+    check_vector(x0, x0.numel(), torch.float64, "x0")
     check(lib().exadg_extrapolate(len(history), _history(history), _ptr(x0), x0.numel(),
                                   _stream(stream)))
 
@@ -442,9 +493,12 @@ class GuessNet:
 
     def loss_and_grad(self, X, T, stream=None):
         import torch
+        _require(X.dim() == 2 and X.shape[1] == self.n_features and X.shape[0] > 0,
+                 f"X must be B x {self.n_features}")
         B = X.shape[0]
-        assert X.dtype == torch.float32 and X.is_contiguous() and X.shape[1] == self.n_features
-        assert T.dtype == torch.float32 and T.is_contiguous() and tuple(T.shape) == (B, self.n_outputs)
+        _require(tuple(T.shape) == (B, self.n_outputs), f"T must be {B} x {self.n_outputs}")
+        check_vector(X, X.numel(), torch.float32, "X")
+        check_vector(T, T.numel(), torch.float32, "T")
         g = torch.empty(self.n_params, dtype=torch.float32, device=X.device)
         loss = ctypes.c_double()
         check(lib().exadg_guess_loss_and_grad(self._h, _ptr(X), _ptr(T), B, _ptr(g),
@@ -456,9 +510,7 @@ class GuessNet:
               seed=7, stream=None):
         """Adam on the FP64 sequence P, Bv (steps x n_dofs, device). loss: GUESS_LOSS_* (default
         ENERGY with an operator, else L2). Returns the last epoch's normalized loss."""
-        import torch
-        assert P.dtype == torch.float64 and Bv.dtype == torch.float64 and P.shape == Bv.shape
-        assert P.is_contiguous() and Bv.is_contiguous()
+        check_sequence(P, Bv, t_begin, t_end, self.history)
         if loss is None:
             loss = GUESS_LOSS_ENERGY if op is not None else GUESS_LOSS_L2
         out = ctypes.c_double()
@@ -468,8 +520,7 @@ class GuessNet:
         return out.value
 
     def evaluate(self, P, Bv, t_begin, t_end, op=None, loss=None, batch_cells=4096, stream=None):
-        import torch
-        assert P.dtype == torch.float64 and Bv.dtype == torch.float64 and P.shape == Bv.shape
+        check_sequence(P, Bv, t_begin, t_end, self.history)
         if loss is None:
             loss = GUESS_LOSS_ENERGY if op is not None else GUESS_LOSS_L2
         out = ctypes.c_double()
@@ -481,9 +532,15 @@ class GuessNet:
     def predict(self, history, rhs, x0, stream=None):
         """x0 from the FP64 history (newest first) and rhs; returns True if the network output was
         non-finite and the plain extrapolation was used."""
-        assert len(history) == self.history
+        import torch
+        _require(len(history) == self.history, f"{self.history} history vectors expected")
         n3 = self.n_outputs
-        assert x0.numel() % n3 == 0
+        _require(x0.numel() % n3 == 0 and x0.numel() > 0,
+                 f"x0: {x0.numel()} entries is not a whole number of {n3}-DoF cells")
+        check_vector(x0, x0.numel(), torch.float64, "x0")
+        check_vector(rhs, x0.numel(), torch.float64, "rhs")
+        for t in history:
+            check_vector(t, x0.numel(), torch.float64, "history vector")
         fb = ctypes.c_int()
         check(lib().exadg_guess_predict(self._h, _history(history), _ptr(rhs), _ptr(x0),
                                         x0.numel() // n3, ctypes.byref(fb), _stream(stream)))

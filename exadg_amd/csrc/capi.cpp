@@ -70,6 +70,7 @@ struct exadg_guess
 struct exadg_preconditioner
 {
   int                                        kind = 0;
+  int64_t                                    n    = 0;  // size of the operator it was built for
   std::unique_ptr<ExaDG::PreconditionerBase> p;
   ExaDG::DeviceVector<double>                work_r, work_c;  // exadg_preconditioner_smooth
 };
@@ -226,6 +227,7 @@ int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_des
     auto       h = std::make_unique<exadg_preconditioner>();
     hipStream_t const s = stream_of(stream);
     h->kind             = kind;
+    h->n                = A.m();
     if(kind == EXADG_PRECOND_NONE)
     {
     }
@@ -285,6 +287,8 @@ int exadg_preconditioner_smooth(exadg_laplace * op, exadg_preconditioner * p, co
     auto &            A = fp64_operator(op);
     hipStream_t const s = stream_of(stream);
     int64_t const     n = A.m();
+    require(p->n == n, "the preconditioner was built for an operator of another size");
+    require(b != x, "b and x must not alias");
     p->work_r.reinit(n);
     p->work_c.reinit(n);
     double * r = p->work_r.data();
@@ -587,6 +591,8 @@ int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double
     require(op && b && x && data, "null argument");
     require(data->max_iter >= 0 && data->rel_tol >= 0 && data->abs_tol >= 0, "invalid solver data");
     auto & A = fp64_operator(op);
+    require(prec == nullptr || prec->n == A.m(),
+            "the preconditioner was built for an operator of another size");
     if(!op->cg)
       op->cg = std::make_unique<ExaDG::Krylov::SolverCG>();
     ExaDG::Krylov::SolverData sd;
@@ -605,17 +611,25 @@ int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double
       g_last_error = "CG reached the iteration limit";
       status       = EXADG_ERR_NO_CONVERGE;
     }
+    else if(!si.true_residual_ok)
+    {
+      g_last_error = "CG converged on the recursive residual, but the true residual ||b - A x|| "
+                     "exceeds 10x the tolerance";
+      status       = EXADG_ERR_TRUE_RESIDUAL;
+    }
   });
   if(info)
   {
     info->iterations = si.iterations;
-    info->converged  = si.converged ? 1 : 0;
+    info->converged  = (si.converged && si.true_residual_ok && !si.not_finite) ? 1 : 0;
     info->not_finite = si.not_finite ? 1 : 0;
     info->r0         = si.r0;
     info->r_final    = si.r_final;
     info->b_norm     = si.b_norm;
     info->rho        = si.rho;
     info->n10        = si.n10;
+    info->r_true     = si.r_true;
+    info->tolerance  = si.tolerance;
   }
   return rc != EXADG_OK ? rc : status;
 }

@@ -20,9 +20,7 @@ SolveInfo SolverCG::solve(Poisson::LaplaceOperator<double> const & A, Preconditi
   SolveInfo info;
 
   // r = b - A x, z = P r
-  A.vmult(v, x, s);
-  vec::copy(r, b, n, s);
-  vec::axpby(r, -1.0, v, 1.0, n, s);
+  A.residual(r, x, b, s);
   if(P)
     P->vmult(z, r, s);
   vec::norm2_and_dot(red_, 1, 2, r, z, n, s);
@@ -37,6 +35,8 @@ SolveInfo SolverCG::solve(Poisson::LaplaceOperator<double> const & A, Preconditi
     return info;
   }
   double const tol = std::max(data.abs_tol, data.rel_tol * (data.rel_mode ? info.b_norm : info.r0));
+  info.tolerance   = tol;
+  info.r_true      = info.r0;
   if(info.r0 <= tol)
   {
     info.converged = true;
@@ -67,6 +67,17 @@ SolveInfo SolverCG::solve(Poisson::LaplaceOperator<double> const & A, Preconditi
       break;
     }
     vec::cg_direction(p, z, red_.result() + new_slot, red_.result() + old_slot, n, s);
+  }
+  if(!info.not_finite)
+  {
+    // final true residual in FP64: one fused residual kernel and a norm
+    A.residual(v, x, b, s);
+    vec::dot(red_, 5, v, v, n, s);
+    info.r_true = std::sqrt(red_.fetch(s, 6)[5]);
+    if(!std::isfinite(info.r_true))
+      info.not_finite = true;
+    else
+      info.true_residual_ok = info.r_true <= 10.0 * tol;
   }
   if(info.iterations > 0 && info.r0 > 0 && info.r_final > 0)
   {

@@ -25,6 +25,10 @@ struct SolveInfo
   int    iterations = 0;
   double r0 = 0, r_final = 0, b_norm = 0;
   bool   converged = false, not_finite = false;
+  // final FP64 true residual ||b - A x|| and whether it is within 10x the stopping tolerance
+  // (the recursively updated residual drifts from it when the preconditioner is inexact)
+  double r_true = 0, tolerance = 0;
+  bool   true_residual_ok = true;
   // performance metrics (iterative_solvers_dealii_wrapper.h:63-78)
   double rho = 0, n10 = 0;
 };

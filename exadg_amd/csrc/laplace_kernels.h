@@ -15,7 +15,9 @@ enum class ApplyMode : int
   diagonal  = 2,  // dst[cell, i] = (A_KK)_{ii}: unit-vector probing of the cell-local block
   // Chebyshev-Jacobi step fused into the epilogue (src = x, dst = x_new):
   // x_new = (1 + f1) x - f1 x_old (if has_old) + f2 dinv (b - A x); x_old may alias dst
-  chebyshev = 3
+  chebyshev = 3,
+  // residual in the epilogue: dst = rhs - A src (dst may alias rhs, not src)
+  residual = 4
 };
 
 // Centrosymmetric n x n matrices (A_ij = A_{n-1-i,n-1-j}, such as the 1D mass and stiffness
@@ -59,7 +61,7 @@ struct SipgArgs
   int64_t        n_cells;
   int            probe;      // diagonal mode: local index of the unit vector
   int            tile_rows;  // traversal: y-rows per tile (one tile per XCD)
-  // chebyshev mode only
+  // chebyshev and residual modes
   Number const * x_old   = nullptr;
   Number const * rhs     = nullptr;
   Number const * dinv    = nullptr;

@@ -662,6 +662,9 @@ __global__ void __launch_bounds__(256)
     else if constexpr(mode == ApplyMode::vmult_add)
       for(int e = lt; e < N3; e += N2)
         a.dst[cell * N3 + e] += A[e];
+    else if constexpr(mode == ApplyMode::residual)
+      for(int e = lt; e < N3; e += N2)
+        a.dst[cell * N3 + e] = a.rhs[cell * N3 + e] - A[e];
     else if constexpr(mode == ApplyMode::chebyshev)
       for(int e = lt; e < N3; e += N2)
       {
@@ -985,6 +988,8 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
         a.dst[g] = ax;
       else if constexpr(mode == ApplyMode::vmult_add)
         a.dst[g] += ax;
+      else if constexpr(mode == ApplyMode::residual)
+        a.dst[g] = a.rhs[g] - ax;
       else
       {
         Number r = (Number(1) + a.f1) * own[k] + a.f2 * a.dinv[g] * (a.rhs[g] - ax);
@@ -1072,6 +1077,7 @@ hipError_t launch_sipg_general(int degree, ApplyMode mode, SipgArgs<Number> cons
     case ApplyMode::vmult_add: return dispatch_general<Number, ApplyMode::vmult_add>(degree, a, s);
     case ApplyMode::diagonal: return dispatch_general<Number, ApplyMode::diagonal>(degree, a, s);
     case ApplyMode::chebyshev: return dispatch_general<Number, ApplyMode::chebyshev>(degree, a, s);
+    case ApplyMode::residual: return dispatch_general<Number, ApplyMode::residual>(degree, a, s);
   }
   return hipErrorInvalidValue;
 }
@@ -1087,6 +1093,7 @@ hipError_t launch_sipg_cartesian(int degree, ApplyMode mode, SipgArgs<Number> co
     case ApplyMode::diagonal: return dispatch_cartesian<Number, ApplyMode::diagonal>(degree, a, s);
     case ApplyMode::chebyshev:
       return dispatch_cartesian<Number, ApplyMode::chebyshev>(degree, a, s);
+    case ApplyMode::residual: return dispatch_cartesian<Number, ApplyMode::residual>(degree, a, s);
   }
   return hipErrorInvalidValue;
 }

@@ -190,6 +190,19 @@ void LaplaceOperator<Number>::chebyshev_step(Number * x_new, Number const * x, N
 }
 
 template<typename Number>
+void LaplaceOperator<Number>::residual(Number * dst, Number const * src, Number const * b,
+                                       hipStream_t s) const
+{
+  if(dst == nullptr || src == nullptr || b == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  if((void const *)dst == (void const *)src || (void const *)b == (void const *)src)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "dst and b must not alias src");
+  SipgArgs<Number> a = make_args(dst, src, 0);
+  a.rhs              = b;
+  launch(ApplyMode::residual, a, s);
+}
+
+template<typename Number>
 void LaplaceOperator<Number>::vmult(Number * dst, Number const * src, hipStream_t s) const
 {
   apply(ApplyMode::vmult, dst, src, 0, s);

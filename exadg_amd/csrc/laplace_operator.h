@@ -76,6 +76,8 @@ public:
   void chebyshev_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
                       Number const * dinv, Number f1, Number f2, bool has_old,
                       hipStream_t s) const;
+  // dst = b - A src in one kernel (the residual in the vmult epilogue); dst may alias b
+  void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const;
 
   void vmult(void * dst, void const * src, hipStream_t s) const override
   {

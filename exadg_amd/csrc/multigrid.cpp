@@ -147,6 +147,10 @@ public:
   {
     op_->calculate_inverse_diagonal(d, s);
   }
+  void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const override
+  {
+    op_->residual(dst, src, b, s);
+  }
   bool chebyshev_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
                       Number const * dinv, Number f1, Number f2, bool has_old,
                       hipStream_t s) const override
@@ -159,6 +163,16 @@ private:
   std::unique_ptr<Poisson::LaplaceOperator<Number>> op_;
 };
 }  // namespace
+
+template<typename Number>
+void LevelOperator<Number>::residual(Number * dst, Number const * src, Number const * b,
+                                     hipStream_t s) const
+{
+  if(dst == b)
+    EXADG_THROW(EXADG_ERR_INTERNAL, "the unfused level residual needs dst != b");
+  vmult(dst, src, s);
+  vec::axpby(dst, Number(1), b, Number(-1), n(), s);  // dst = b - A src
+}
 
 // ---------------------------------------------------------------- Jacobi
 
@@ -655,8 +669,7 @@ void MultigridPreconditioner<Number>::v_cycle(int l, hipStream_t s)
   Level &       L = levels_[l];
   int64_t const n = L.op->n();
   L.smoother.vmult(L.sol, L.sol_alt, L.defect.data(), s);
-  L.op->vmult(L.t.data(), L.sol, s);
-  vec::axpby(L.t.data(), Number(1), L.defect.data(), Number(-1), n, s);  // t = defect - A sol
+  L.op->residual(L.t.data(), L.sol, L.defect.data(), s);  // t = defect - A sol
   restrict_and_add(l, levels_[l - 1].defect.data(), L.t.data(), s);
   v_cycle(l - 1, s);
   prolongate_and_add(l, L.sol, levels_[l - 1].sol, s);
@@ -758,6 +771,8 @@ std::unique_ptr<PreconditionerBase> create_multigrid(std::shared_ptr<StructuredG
   EXADG_THROW(EXADG_ERR_ARGUMENT, "multigrid precision must be EXADG_FP64 or EXADG_FP32");
 }
 
+template class LevelOperator<double>;
+template class LevelOperator<float>;
 template class Q1Operator<double>;
 template class Q1Operator<float>;
 template class ChebyshevSmoother<double>;

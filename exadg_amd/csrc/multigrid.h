@@ -92,6 +92,9 @@ public:
   virtual void    vmult(Number * dst, Number const * src, hipStream_t s) const = 0;
   virtual void    inverse_diagonal(Number * d, hipStream_t s) const           = 0;
   virtual bool    continuous() const { return false; }
+  // dst = b - A src; operators with a fused epilogue override this (the default applies A and
+  // then updates, so it needs dst != b)
+  virtual void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const;
   // mask of constrained entries (1 constrained, 0 free) on the host; empty if none
   virtual std::vector<char> constrained_mask() const { return {}; }
   // x_new = (1+f1) x - f1 x_old (if has_old) + f2 dinv (b - A x) in one pass; false if this

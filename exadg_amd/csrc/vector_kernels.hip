@@ -18,15 +18,42 @@ inline int grid_for(int64_t n, int cap = 8192)
   return (int)std::min<int64_t>(std::max<int64_t>(b, 1), cap);
 }
 
-// block-wide sum of one double per thread (wave64 shuffles, then LDS across waves)
+// Wave64 sum of one double per lane by DPP row permutations: two quad swaps, the half-row
+// mirror and the row mirror leave every lane of a row of 16 holding that row's sum (each step
+// adds a lane's value to its partner's, so both partners hold the same bits); the four row
+// sums are then read from lanes 0, 16, 32, 48 and added in that order. Deterministic.
+template<int Ctrl>
+__device__ __forceinline__ double dpp_move(double v)
+{
+  int const lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), Ctrl, 0xf, 0xf, true);
+  int const hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), Ctrl, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+  int const lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  int const hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return ((read_lane(v, 0) + read_lane(v, 16)) + read_lane(v, 32)) + read_lane(v, 48);
+}
+
+// block-wide sum of one double per thread (DPP within each wave, then LDS across the waves);
+// callers separate two uses by a barrier
 __device__ __forceinline__ double block_sum(double v)
 {
   __shared__ double wsum[kBlock / 64];
-#pragma unroll
-  for(int off = 32; off > 0; off >>= 1)
-    v += __shfl_down(v, off, 64);
-  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if(lane == 0)
+  v              = wave_sum(v);
+  int const wave = threadIdx.x >> 6;
+  if((threadIdx.x & 63) == 0)
     wsum[wave] = v;
   __syncthreads();
   double r = 0;
@@ -36,11 +63,15 @@ __device__ __forceinline__ double block_sum(double v)
   return r;  // valid on thread 0
 }
 
+// One-launch deterministic reduction: every block writes its partial sum(s), and the block
+// that finishes last (ticket from a device-scope counter) adds the partials in block order,
+// writes the result slot(s) and resets the counter to zero for the next reduction.
+// K = 0: out0 = sum a*b; K = 1: out0 = sum a*b and out1 = sum a*a.
 template<typename Number, int K>
 __global__ void __launch_bounds__(kBlock)
-  reduce_stage1(Number const * a, Number const * b, int64_t n, double * partials)
+  reduce_fused(Number const * a, Number const * b, int64_t n, double * partials,
+               unsigned int * counter, double * out0, double * out1)
 {
-  // K = 0: sum a*b ; K = 1: sum a*a and sum a*b (two partial arrays)
   double        s0 = 0, s1 = 0;
   int64_t const stride = (int64_t)gridDim.x * kBlock;
   for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
@@ -51,15 +82,45 @@ __global__ void __launch_bounds__(kBlock)
     if constexpr(K == 1)
       s1 += x * x;
   }
-  double const r0 = block_sum(s0);
+  __shared__ bool last;
+  double const    r0 = block_sum(s0);
   __syncthreads();
-  if(threadIdx.x == 0)
-    partials[blockIdx.x] = r0;
+  double r1 = 0;
   if constexpr(K == 1)
   {
-    double const r1 = block_sum(s1);
-    if(threadIdx.x == 0)
+    r1 = block_sum(s1);
+    __syncthreads();
+  }
+  if(threadIdx.x == 0)
+  {
+    partials[blockIdx.x] = r0;
+    if constexpr(K == 1)
       partials[Reducer::kMaxPartials + blockIdx.x] = r1;
+    __threadfence();
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if(!last)
+    return;
+  __threadfence();
+  double volatile const * pv = partials;
+  double                  t0 = 0, t1 = 0;
+  for(int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+  {
+    t0 += pv[i];
+    if constexpr(K == 1)
+      t1 += pv[Reducer::kMaxPartials + i];
+  }
+  t0 = block_sum(t0);
+  __syncthreads();
+  if constexpr(K == 1)
+    t1 = block_sum(t1);
+  if(threadIdx.x == 0)
+  {
+    *out0 = t0;
+    if constexpr(K == 1)
+      *out1 = t1;
+    *counter = 0u;
   }
 }
 
@@ -204,6 +265,8 @@ Reducer::Reducer()
 {
   partials_.resize(2 * kMaxPartials);
   result_.resize(kSlots);
+  counter_.resize(1);
+  HIP_CHECK(hipMemset(counter_.data(), 0, sizeof(unsigned int)));
   HIP_CHECK(hipHostMalloc((void **)&host_, kSlots * sizeof(double), hipHostMallocDefault));
 }
 
@@ -254,10 +317,8 @@ template<typename Number>
 void dot(Reducer & red, int slot, Number const * a, Number const * b, int64_t n, hipStream_t s)
 {
   int const g = grid_for(n, Reducer::kMaxPartials);
-  hipLaunchKernelGGL((reduce_stage1<Number, 0>), dim3(g), dim3(kBlock), 0, s, a, b, n,
-                     red.partials());
-  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, red.partials(), g,
-                     red.result() + slot);
+  hipLaunchKernelGGL((reduce_fused<Number, 0>), dim3(g), dim3(kBlock), 0, s, a, b, n,
+                     red.partials(), red.counter(), red.result() + slot, nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -266,12 +327,8 @@ void norm2_and_dot(Reducer & red, int slot_aa, int slot_ab, Number const * a, Nu
                    int64_t n, hipStream_t s)
 {
   int const g = grid_for(n, Reducer::kMaxPartials);
-  hipLaunchKernelGGL((reduce_stage1<Number, 1>), dim3(g), dim3(kBlock), 0, s, a, b, n,
-                     red.partials());
-  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s, red.partials(), g,
-                     red.result() + slot_ab);
-  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kBlock), 0, s,
-                     red.partials() + Reducer::kMaxPartials, g, red.result() + slot_aa);
+  hipLaunchKernelGGL((reduce_fused<Number, 1>), dim3(g), dim3(kBlock), 0, s, a, b, n,
+                     red.partials(), red.counter(), red.result() + slot_ab, red.result() + slot_aa);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -29,8 +29,10 @@ private:
   DeviceBuffer<Number> buf_;
 };
 
-// Reduction workspace: partial sums per block and the final scalars (device), plus a pinned
-// host mirror for the values the host needs (convergence checks).
+// Reduction workspace: partial sums per block, the final scalars and the ticket counter of the
+// one-launch reduction (device), plus a pinned host mirror for the values the host needs
+// (convergence checks). Reductions on one Reducer must be stream-ordered (they share the
+// partials and the counter).
 class Reducer
 {
 public:
@@ -42,11 +44,13 @@ public:
   Reducer & operator=(Reducer const &) = delete;
   double * partials() { return partials_.data(); }
   double * result() { return result_.data(); }  // device, kSlots doubles
+  unsigned int * counter() { return counter_.data(); }  // device, zero between reductions
   // copy the result slots to the host and wait
   double const * fetch(hipStream_t s, int n_slots = kSlots);
 
 private:
-  DeviceBuffer<double> partials_, result_;
+  DeviceBuffer<double>       partials_, result_;
+  DeviceBuffer<unsigned int> counter_;
   double *             host_ = nullptr;
 };
 

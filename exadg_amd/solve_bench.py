@@ -30,7 +30,8 @@ def run_solve(ex, torch, mesh, bc, degree, degrees, omega, coarse_solver, precis
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     return dict(n_dofs=op.n, iterations=info["iterations"], time_s=min(times), setup_s=t_setup,
-                rel_residual=info["r_final"] / info["b_norm"]), x, op
+                rel_residual=info["r_final"] / info["b_norm"],
+                true_rel_residual=info["r_true"] / info["b_norm"]), x, op
 
 
 def compare_arms(ex, torch, mesh, bc, degree, degrees, omega, coarse_solver, reps=1):

@@ -30,7 +30,9 @@ enum exadg_status
   EXADG_ERR_NOT_FINITE   = 3, /* NaN/Inf detected (AssertThrow(std::isfinite) in the ref.) */
   EXADG_ERR_NO_CONVERGE  = 4, /* iteration limit reached (SolverControl::NoConvergence)     */
   EXADG_ERR_NOT_READY    = 5, /* object needs update() before use                           */
-  EXADG_ERR_INTERNAL     = 6
+  EXADG_ERR_INTERNAL     = 6,
+  EXADG_ERR_TRUE_RESIDUAL = 7 /* solver converged on its recursive residual, but the final FP64
+                                 true residual ||b - A x|| exceeds 10x the tolerance          */
 };
 
 enum exadg_precision
@@ -155,7 +157,10 @@ typedef struct exadg_preconditioner exadg_preconditioner;
 /* The preconditioner factory of Poisson::Operator::setup_preconditioner_and_solver
  * (operator.cpp:306-398) for None / PointJacobi / Multigrid; `mg` is read for MULTIGRID only.
  * The operator must be FP64 (the outer solver's precision). Setup includes the diagonals,
- * the Chebyshev eigenvalue estimates and the coarse solver (PreconditionerBase::update). */
+ * the Chebyshev eigenvalue estimates and the coarse solver (PreconditionerBase::update).
+ * Lifetime: a POINT_JACOBI preconditioner refers to `op` (as the reference's
+ * JacobiPreconditioner holds a reference to its operator, jacobi_preconditioner.h:33-82), so
+ * `op` must outlive it; a MULTIGRID one shares the operator's grid and owns its levels. */
 int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_desc * mg,
                                 exadg_preconditioner ** out, void * stream);
 /* PreconditionerBase::vmult (preconditioner_base.h:33-67): dst = P^{-1} src, FP64 vectors. */
@@ -195,12 +200,17 @@ typedef struct exadg_solve_info
   double  b_norm;
   double  rho; /* average convergence rate (iterative_solvers_dealii_wrapper.h:63-78) */
   double  n10; /* iterations per 10 orders of magnitude */
+  double  r_true;    /* final true residual ||b - A x||, FP64 (one residual kernel + norm)  */
+  double  tolerance; /* the stopping tolerance max(abs_tol, rel_tol * ||r0|| or ||b||)      */
 } exadg_solve_info;
 
 /* Krylov::KrylovSolver::solve with CG (iterative_solvers_dealii_wrapper.h:136-221) on the FP64
- * operator, preconditioned by `prec` (NULL = none); x holds the initial guess on entry.
- * Returns EXADG_ERR_NOT_FINITE on NaN/Inf and EXADG_ERR_NO_CONVERGE when max_iter is hit
- * (info is filled in both cases). */
+ * operator, preconditioned by `prec` (NULL = none; it must have been created for `op` or for an
+ * operator of the same size, else EXADG_ERR_ARGUMENT); x holds the initial guess on entry.
+ * Returns EXADG_ERR_NOT_FINITE on NaN/Inf (the NaN check at :210-211), EXADG_ERR_NO_CONVERGE
+ * when max_iter is hit, and EXADG_ERR_TRUE_RESIDUAL when the final FP64 true residual
+ * ||b - A x|| exceeds 10x the tolerance although the recursive residual met it (info is filled
+ * in every case; converged = 0 in all three). */
 int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
                    const exadg_solver_data * data, exadg_solve_info * info, void * stream);
 

@@ -53,6 +53,28 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_native_build(out_dir):
+    """Build a copy of the oracle tuned for the host it runs on (-march=native) into out_dir and
+    use it from now on: the CPU-baseline leg of bench.py, on the measuring host. Must be called
+    before the library is first loaded. Returns the library path, or None if the build failed
+    (the portable x86-64-v3 build is then used)."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    path = os.path.join(out_dir, "liboracle_native.so")
+    r = subprocess.run(["make", "-s", "-B", "-C", _HERE, f"OUT={path}", "ARCH=-march=native"],
+                       capture_output=True, text=True)
+    if r.returncode != 0 or not os.path.exists(path):
+        return None
+    _LIB_PATH = path
+    return path
+
+
+def num_threads():
+    """OpenMP threads of the oracle's parallel loops."""
+    return lib().oracle_num_threads()
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -78,6 +100,8 @@ def lib():
                                                    ctypes.c_int, dp, dp, dp, dp]
         L.oracle_tables.argtypes = [ctypes.c_int] + [dp] * 7
         L.oracle_random_vector.argtypes = [ctypes.c_longlong, ctypes.c_ulonglong, dp]
+        L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_num_threads.argtypes = []
         L.oracle_cg_solve.restype = ctypes.c_int
         L.oracle_cg_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, dp]

@@ -5,6 +5,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "oracle_core.h"
 #include "oracle_mg.h"
 #include "oracle_solver.h"
@@ -14,6 +16,9 @@ using namespace oracle;
 static thread_local std::string g_err;
 
 extern "C" {
+
+// OpenMP threads the oracle's parallel loops use (the CPU baseline's core count)
+int oracle_num_threads(void) { return omp_get_max_threads(); }
 
 struct oracle_mesh
 {

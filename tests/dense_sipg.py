@@ -135,8 +135,11 @@ def _face_geo(mesh, cc, d, s, gp, gw):
     return xi, x, Ji, det * nrm * w, nv / nrm[:, None]
 
 
-def assemble(mesh, degree, ip=1.0, omega=None):
-    """dense A (and b for u = prod sin(omega x) if omega is given)"""
+def assemble(mesh, degree, ip=1.0, omega=None, plant=None):
+    """dense A (and b for u = prod sin(omega x) if omega is given). `plant` assembles a
+    deliberately wrong operator for the tests that show a comparison catches plausible
+    mistakes: "drop_symmetry" (no -[u]{dv/dn} term on interior faces), "penalty_min" (min
+    instead of max of the cell penalties on interior faces)."""
     n = degree + 1
     n3 = n ** 3
     b1 = Basis1D(gl_nodes(n))
@@ -183,8 +186,10 @@ def assemble(mesh, degree, ip=1.0, omega=None):
                 im, ip_ = mesh.index(cc) * n3, mesh.index(nb) * n3
                 jmp = np.concatenate([vm, -vp], axis=1)
                 avg = 0.5 * np.concatenate([dm, dp], axis=1)
-                t = pf * max(tau[cc], tau[nb])
-                loc = np.einsum("q,qa,qb->ab", fj, -jmp, avg) + np.einsum("q,qa,qb->ab", fj, -avg, jmp) \
+                t = pf * (min if plant == "penalty_min" else max)(tau[cc], tau[nb])
+                sym = 0.0 if plant == "drop_symmetry" else 1.0
+                loc = np.einsum("q,qa,qb->ab", fj, -jmp, avg) \
+                    + sym * np.einsum("q,qa,qb->ab", fj, -avg, jmp) \
                     + t * np.einsum("q,qa,qb->ab", fj, jmp, jmp)
                 # block-wise so that a cell that is its own periodic neighbour accumulates
                 A[im:im + n3, im:im + n3] += loc[:n3, :n3]

@@ -70,7 +70,8 @@ def test_extrapolate(torch_cuda, product, order):
     assert np.max(np.abs(x0.cpu().numpy() - ref)) < 1e-15 * 8
 
 
-@pytest.mark.parametrize("degree,history,width,B", [(1, 3, 40, 70), (2, 3, 72, 133), (3, 2, 128, 300)])
+@pytest.mark.parametrize("degree,history,width,B", [(1, 3, 40, 70), (2, 3, 72, 133), (3, 2, 128, 300),
+                                                    (4, 3, 128, 257), (5, 3, 128, 190)])
 def test_loss_and_grad_vs_oracle(torch_cuda, product, degree, history, width, B):
     torch = torch_cuda
     net = product.GuessNet(degree, history=history, width=width, seed=11)

@@ -18,7 +18,8 @@ def _cases():
     for k in range(1, 9):
         cases.append(("cart", k, (3, 2, 4), (0.0, -0.5, 0.2), (1.0, 0.5, 1.4), 0.0, 1, 1,
                       (0, 0, 0), (D, N, D, D, N, D)))
-        cases.append(("curved", k, (3, 3, 2), 0.0, 1.0, 0.1, 1, min(k, 4), (0, 0, 0),
+        # MappingQ(k), the mapping of the curved operator sweep and the mixed-precision solve
+        cases.append(("curved", k, (3, 3, 2), 0.0, 1.0, 0.1, 1, k, (0, 0, 0),
                       (D, D, N, D, D, D)))
     cases.append(("periodic_xy", 4, (4, 3, 3), 0.0, 1.0, 0.0, 1, 1, (1, 1, 0), (D,) * 4 + (N, N)))
     cases.append(("periodic_curved", 3, (3, 2, 3), 0.0, 1.0, 0.08, 1, 3, (1, 0, 1), (D,) * 6))

@@ -56,15 +56,20 @@ def test_oracle_vmult_equals_dense_assembly(case):
         assert np.abs(om.vmult(np.ones(om.n))).max() <= 1e-11 * np.abs(A).max()
 
 
-def test_dense_assembly_detects_planted_sign_error():
-    """A dropped symmetry term (a plausible mistake) must break the dense comparison."""
-    om, dm, k = _pair(CASES[1])
+@pytest.mark.parametrize("plant", ["drop_symmetry", "penalty_min"])
+def test_dense_comparison_detects_planted_mistakes(plant):
+    """Plausible mistakes planted in the dense assembly (the interior-face symmetry term
+    -[u]{dv/dn} dropped; min instead of max of the cell penalties) are caught: the oracle
+    matches the correct matrix to 1e-12 and misses the planted one by far more."""
+    # cells of different penalty (boundary cells have larger tau) for "penalty_min"
+    om, dm, k = _pair(((3, 1, 2), 0.0, 1.0, 0.0, 1, 1, (0, 0, 0), (D, N, D, D, N, D), 2))
     A, _ = dense_sipg.assemble(dm, k)
+    Ap, _ = dense_sipg.assemble(dm, k, plant=plant)
+    assert np.abs(Ap - A).max() > 1e-3 * np.abs(A).max()
     x = oracle.random_vector(om.n)
     y = om.vmult(x)
-    # remove the -[u]{dv/dn} term by assembling the non-symmetric form: compare with A - sym part
-    ev_diff = np.linalg.norm(y - 0.9 * (A @ x))
-    assert ev_diff > 1e-3 * np.linalg.norm(y)
+    assert np.linalg.norm(y - A @ x) <= 1e-12 * np.linalg.norm(y)
+    assert np.linalg.norm(y - Ap @ x) > 1e-4 * np.linalg.norm(y)
 
 
 def test_cartesian_penalty_closed_form():
