@@ -141,6 +141,9 @@ def lib():
             "exadg_laplace_calculate_inverse_diagonal": [vp, vp, vp],
             "exadg_laplace_calculate_diagonal": [vp, vp, vp],
             "exadg_laplace_rhs_sine": [vp, ctypes.c_double, vp, vp],
+            "exadg_laplace_rhs_n_points": [vp, i64p, i64p],
+            "exadg_laplace_rhs_points": [vp, vp, vp, vp],
+            "exadg_laplace_rhs": [vp, vp, vp, vp, vp],
             "exadg_mg_desc_default": [ctypes.c_int, ctypes.POINTER(MGDesc)],
             "exadg_preconditioner_create": [vp, ctypes.c_int, ctypes.POINTER(MGDesc),
                                             ctypes.POINTER(vp), vp],
@@ -288,6 +291,35 @@ class LaplaceOperator:
         import torch
         check_vector(b, self.n, torch.float64, "b")
         check(lib().exadg_laplace_rhs_sine(self._h, omega, _ptr(b), _stream(stream)))
+
+    def rhs_n_points(self):
+        """(cell points, boundary points) at which rhs() takes its data."""
+        nc, nb = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().exadg_laplace_rhs_n_points(self._h, ctypes.byref(nc), ctypes.byref(nb)))
+        return nc.value, nb.value
+
+    def rhs_points(self, stream=None):
+        """Coordinates of the right-hand-side points: device tensors (3, n_cell_points) and
+        (3, n_boundary_points), FP64."""
+        import torch
+        nc, nb = self.rhs_n_points()
+        xc = torch.empty((3, nc), dtype=torch.float64, device="cuda")
+        xb = torch.empty((3, max(nb, 0)), dtype=torch.float64, device="cuda")
+        check(lib().exadg_laplace_rhs_points(self._h, _ptr(xc), _ptr(xb) if nb > 0 else None,
+                                             _stream(stream)))
+        return xc, xb
+
+    def rhs(self, b, f=None, boundary=None, stream=None):
+        """b = int f v + Dirichlet (-g dv/dn + 2 tau g v) + Neumann h v from point values (f at the
+        cell points, g / h at the boundary points; None = zero)."""
+        import torch
+        nc, nb = self.rhs_n_points()
+        check_vector(b, self.n, torch.float64, "b")
+        if f is not None:
+            check_vector(f, nc, torch.float64, "f")
+        if boundary is not None:
+            check_vector(boundary, nb, torch.float64, "boundary")
+        check(lib().exadg_laplace_rhs(self._h, _ptr(f), _ptr(boundary), _ptr(b), _stream(stream)))
 
     def error_sine(self, omega, u, stream=None):
         """Relative L2 error against prod sin(omega x_d), QGauss(k+3) (FP64 operator)."""

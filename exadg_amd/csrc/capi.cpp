@@ -190,6 +190,32 @@ int exadg_laplace_rhs_sine(exadg_laplace * op, double omega, double * b, void * 
   });
 }
 
+int exadg_laplace_rhs_n_points(const exadg_laplace * op, int64_t * n_cell_points,
+                               int64_t * n_boundary_points)
+{
+  return guarded([&] {
+    require(op && n_cell_points && n_boundary_points, "null argument");
+    op->op->rhs_n_points(*n_cell_points, *n_boundary_points);
+  });
+}
+
+int exadg_laplace_rhs_points(exadg_laplace * op, double * x_cell, double * x_boundary, void * stream)
+{
+  return guarded([&] {
+    require(op, "null operator");
+    op->op->rhs_points(x_cell, x_boundary, stream_of(stream));
+  });
+}
+
+int exadg_laplace_rhs(exadg_laplace * op, const double * f, const double * boundary, double * b,
+                      void * stream)
+{
+  return guarded([&] {
+    require(op && b, "null argument");
+    op->op->rhs(f, boundary, b, stream_of(stream));
+  });
+}
+
 int exadg_mg_desc_default(int degree, exadg_mg_desc * out)
 {
   return guarded([&] {

@@ -404,3 +404,85 @@ DivergenceGeometry compute_divergence_geometry(StructuredGrid const & grid, Shap
 }
 
 }  // namespace ExaDG
+
+namespace ExaDG
+{
+RhsGeometry compute_rhs_geometry(StructuredGrid const & grid, ShapeInfo1D const & shape,
+                                 std::vector<double> const & tau_face)
+{
+  RhsGeometry         g;
+  int const           n = shape.n, n2 = n * n, n3 = n2 * n;
+  int64_t const       nc = grid.n_cells();
+  BoxTopology const & topo = grid.topology();
+  g.cell.resize((size_t)nc * kRhsCellRec * n3);
+  parallel_range(nc, [&](int64_t b, int64_t e) {
+    std::vector<double> x(3 * n3), J(9 * n3);
+    for(int64_t c = b; c < e; ++c)
+    {
+      grid.evaluate(c, shape.qp, shape.qp, shape.qp, x.data(), J.data());
+      double * out = &g.cell[(size_t)c * kRhsCellRec * n3];
+      for(int q = 0; q < n3; ++q)
+      {
+        double Jq[9], Ji[9];
+        for(int k = 0; k < 9; ++k)
+          Jq[k] = J[k * n3 + q];
+        double const det = inverse3(Jq, Ji);
+        for(int d = 0; d < 3; ++d)
+          out[d * n3 + q] = x[d * n3 + q];
+        out[3 * n3 + q] = std::abs(det) * shape.qw[q % n] * shape.qw[(q / n) % n] * shape.qw[q / n2];
+      }
+    }
+  });
+  // boundary faces, side-major; on a side lexicographic in (t1, t2)
+  g.side_base[0] = 0;
+  for(int side = 0; side < 6; ++side)
+  {
+    int const d = side / 2;
+    g.side_base[side + 1] =
+      g.side_base[side] + (topo.periodic[d] ? 0 : (int64_t)topo.nc[tangent1(d)] * topo.nc[tangent2(d)]);
+  }
+  g.face.resize((size_t)g.side_base[6] * kRhsFaceRec * n2);
+  std::vector<double> const one0 = {0.0}, one1 = {1.0};
+  for(int side = 0; side < 6; ++side)
+  {
+    int const     d = side / 2, s = side % 2, t1 = tangent1(d), t2 = tangent2(d);
+    int64_t const nf = g.side_base[side + 1] - g.side_base[side];
+    parallel_range(nf, [&](int64_t b, int64_t e) {
+      std::vector<double> xf(3 * n2), Jf(9 * n2);
+      for(int64_t fi = b; fi < e; ++fi)
+      {
+        int cc[3];
+        cc[t1] = (int)(fi % topo.nc[t1]);
+        cc[t2] = (int)(fi / topo.nc[t1]);
+        cc[d]  = s ? topo.nc[d] - 1 : 0;
+        std::vector<double> const * p[3] = {&shape.qp, &shape.qp, &shape.qp};
+        p[d]                             = s ? &one1 : &one0;
+        grid.evaluate(cell_id(topo, cc), *p[0], *p[1], *p[2], xf.data(), Jf.data());
+        double const tau = tau_face[face_id(topo, cc, d, s)];
+        double *     out = &g.face[(size_t)(g.side_base[side] + fi) * kRhsFaceRec * n2];
+        for(int q = 0; q < n2; ++q)
+        {
+          double Jq[9], Ji[9];
+          for(int k = 0; k < 9; ++k)
+            Jq[k] = Jf[k * n2 + q];
+          double const det = inverse3(Jq, Ji);
+          // outward unit normal: the gradient of xi_d, oriented away from the cell
+          double       nv[3] = {Ji[d * 3 + 0], Ji[d * 3 + 1], Ji[d * 3 + 2]};
+          double const nrm = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+          for(int k = 0; k < 3; ++k)
+            nv[k] *= (s ? 1.0 : -1.0) / nrm;
+          for(int k = 0; k < 3; ++k)
+          {
+            out[k * n2 + q]       = xf[k * n2 + q];
+            out[(3 + k) * n2 + q] = nv[k];
+            out[(7 + k) * n2 + q] = Ji[k * 3 + 0] * nv[0] + Ji[k * 3 + 1] * nv[1] + Ji[k * 3 + 2] * nv[2];
+          }
+          out[6 * n2 + q]  = std::abs(det) * nrm * shape.qw[q % n] * shape.qw[q / n];
+          out[10 * n2 + q] = tau;
+        }
+      }
+    });
+  }
+  return g;
+}
+}  // namespace ExaDG

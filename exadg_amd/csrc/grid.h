@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.h"
+#include "rhs_kernels.h"
 #include "shape_info.h"
 #include "topology.h"
 
@@ -68,5 +69,18 @@ struct DivergenceGeometry
   std::vector<double> face;
 };
 DivergenceGeometry compute_divergence_geometry(StructuredGrid const & grid, ShapeInfo1D const & shape);
+
+// Point data of the right-hand side assembly (rhs_kernels.h records) on any mesh, FP64: per
+// cell x and JxW at the Gauss points; per boundary face of the non-periodic sides x, the
+// outward unit normal, JxW, J^{-1} n and tau_F at the face Gauss points. Boundary faces are
+// numbered side-major (0:x- 1:x+ 2:y- 3:y+ 4:z- 5:z+), on a side lexicographic in the two
+// tangential cell coordinates (lower direction fastest); side_base[s] is a side's first face.
+struct RhsGeometry
+{
+  std::vector<double> cell, face;
+  int64_t             side_base[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+RhsGeometry compute_rhs_geometry(StructuredGrid const & grid, ShapeInfo1D const & shape,
+                                 std::vector<double> const & tau_face);
 
 }  // namespace ExaDG

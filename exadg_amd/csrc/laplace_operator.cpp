@@ -97,6 +97,22 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
     dt.insert(dt.end(), shape_.qw.begin(), shape_.qw.end());
     dt.insert(dt.end(), shape_.qp.begin(), shape_.qp.end());
     div_tables_.upload(dt);
+    // right-hand side tables: S, Sd, d0, d1
+    std::vector<double> rt(shape_.S);
+    rt.insert(rt.end(), Sd.begin(), Sd.end());
+    rt.insert(rt.end(), shape_.d0.begin(), shape_.d0.end());
+    rt.insert(rt.end(), shape_.d1.begin(), shape_.d1.end());
+    rhs_tables_.upload(rt);
+  }
+  {
+    // boundary faces of the non-periodic sides, side-major (the RhsGeometry numbering)
+    BoxTopology const & t = grid_->topology();
+    for(int side = 0; side < 6; ++side)
+    {
+      int const d = side / 2;
+      rhs_side_base_[side + 1] =
+        rhs_side_base_[side] + (t.periodic[d] ? 0 : (int64_t)t.nc[tangent1(d)] * t.nc[tangent2(d)]);
+    }
   }
   {
     // error norm with QGauss(k+3): Se[q][i] = phi_i(e_q), then weights, then points
@@ -232,14 +248,83 @@ void LaplaceOperator<Number>::calculate_inverse_diagonal(Number * diag, hipStrea
 }
 
 template<typename Number>
-void LaplaceOperator<Number>::rhs_sine(double omega, double * b, hipStream_t s) const
+void LaplaceOperator<Number>::ensure_rhs_geometry() const
+{
+  if(rhs_cell_geo_.size() > 0)
+    return;
+  RhsGeometry const g = compute_rhs_geometry(*grid_, shape_, geo_.tau_face);
+  rhs_cell_geo_.upload(g.cell);
+  if(!g.face.empty())
+    rhs_face_geo_.upload(g.face);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::rhs_launch(double const * f, double const * boundary, int sine,
+                                         double omega, double * b, hipStream_t s) const
 {
   if(b == nullptr)
     EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
-  std::vector<double> hb;
-  assemble_rhs_sine(*grid_, shape_, geo_, omega, hb);
-  HIP_CHECK(hipMemcpyAsync(b, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  ensure_rhs_geometry();
+  RhsArgs a;
+  a.b        = b;
+  a.cell_geo = rhs_cell_geo_.data();
+  a.face_geo = rhs_face_geo_.data();
+  a.f        = f;
+  a.boundary = boundary;
+  a.tables   = rhs_tables_.data();
+  a.topo     = grid_->topology();
+  std::copy(rhs_side_base_, rhs_side_base_ + 7, a.side_base);
+  a.n       = shape_.n;
+  a.sine    = sine;
+  a.omega   = omega;
+  a.n_cells = grid_->n_cells();
+  hipError_t const e = launch_rhs_assemble(a, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, std::string("rhs kernel launch: ") + hipGetErrorString(e));
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::rhs_sine(double omega, double * b, hipStream_t s) const
+{
+  rhs_launch(nullptr, nullptr, 1, omega, b, s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::rhs(double const * f, double const * boundary, double * b,
+                                  hipStream_t s) const
+{
+  if((f != nullptr && (void const *)f == (void const *)b) ||
+     (boundary != nullptr && (void const *)boundary == (void const *)b))
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "b must not alias the point values");
+  rhs_launch(f, boundary, 0, 0.0, b, s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::rhs_n_points(int64_t & n_cell_points, int64_t & n_boundary_points) const
+{
+  int const n     = shape_.n;
+  n_cell_points     = grid_->n_cells() * n3_;
+  n_boundary_points = rhs_side_base_[6] * n * n;
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::rhs_points(double * x_cell, double * x_boundary, hipStream_t s) const
+{
+  ensure_rhs_geometry();
+  int const     n = shape_.n, n2 = n * n;
+  int64_t const nc = grid_->n_cells(), nb = rhs_side_base_[6];
+  // records are SoA per cell / face; the output is component-major over all points
+  for(int d = 0; d < 3; ++d)
+  {
+    if(x_cell)
+      HIP_CHECK(hipMemcpy2DAsync(x_cell + d * nc * n3_, n3_ * sizeof(double),
+                                 rhs_cell_geo_.data() + d * n3_, kRhsCellRec * n3_ * sizeof(double),
+                                 n3_ * sizeof(double), nc, hipMemcpyDeviceToDevice, s));
+    if(x_boundary && nb > 0)
+      HIP_CHECK(hipMemcpy2DAsync(x_boundary + d * nb * n2, n2 * sizeof(double),
+                                 rhs_face_geo_.data() + d * n2, kRhsFaceRec * n2 * sizeof(double),
+                                 n2 * sizeof(double), nb, hipMemcpyDeviceToDevice, s));
+  }
 }
 
 template<typename Number>

@@ -38,6 +38,11 @@ public:
   virtual void    calculate_diagonal(void * diag, hipStream_t s) const         = 0;
   virtual void    calculate_inverse_diagonal(void * diag, hipStream_t s) const = 0;
   virtual void    rhs_sine(double omega, double * b, hipStream_t s) const      = 0;
+  // right-hand side from point values (f at the cell Gauss points, g / h at the boundary-face
+  // Gauss points; NULL = zero) and the points themselves (RhsGeometry numbering)
+  virtual void rhs(double const * f, double const * boundary, double * b, hipStream_t s) const = 0;
+  virtual void rhs_n_points(int64_t & n_cell_points, int64_t & n_boundary_points) const     = 0;
+  virtual void rhs_points(double * x_cell, double * x_boundary, hipStream_t s) const        = 0;
   // weak DG divergence (central flux, no-slip walls) of a Fourier velocity field; mean
   // removed when the operator is singular
   virtual void rhs_divergence_fourier(double const * modes_host, int n_modes, double t, double * b,
@@ -95,8 +100,11 @@ public:
   {
     calculate_inverse_diagonal(static_cast<Number *>(d), s);
   }
-  // Poisson::Operator::rhs for the manufactured sine solution (FP64 output)
+  // Poisson::Operator::rhs for the manufactured sine solution (FP64 output), on the device
   void rhs_sine(double omega, double * b, hipStream_t s) const override;
+  void rhs(double const * f, double const * boundary, double * b, hipStream_t s) const override;
+  void rhs_n_points(int64_t & n_cell_points, int64_t & n_boundary_points) const override;
+  void rhs_points(double * x_cell, double * x_boundary, hipStream_t s) const override;
   void rhs_divergence_fourier(double const * modes_host, int n_modes, double t, double * b,
                               hipStream_t s) const override;
   bool   is_singular() const override;
@@ -112,6 +120,10 @@ private:
   void apply(ApplyMode mode, Number * dst, Number const * src, int probe, hipStream_t s) const;
   SipgArgs<Number> make_args(Number * dst, Number const * src, int probe) const;
   void             launch(ApplyMode mode, SipgArgs<Number> const & a, hipStream_t s) const;
+  // point data of the right-hand side, computed on first use
+  void rhs_launch(double const * f, double const * boundary, int sine, double omega, double * b,
+                  hipStream_t s) const;
+  void ensure_rhs_geometry() const;
 
   std::shared_ptr<StructuredGrid const> grid_;
   LaplaceOperatorData                   data_;
@@ -122,6 +134,9 @@ private:
   Number                                h_[6] = {1, 1, 1, 1, 1, 1};
   DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
   DeviceBuffer<double>                  div_tables_;          // S, Sd, weights|points (FP64)
+  DeviceBuffer<double>                  rhs_tables_;          // S, Sd, d0, d1 (FP64)
+  mutable DeviceBuffer<double>          rhs_cell_geo_, rhs_face_geo_;
+  int64_t                               rhs_side_base_[7] = {0, 0, 0, 0, 0, 0, 0};
   mutable DeviceBuffer<double>          div_cell_geo_, div_face_geo_;  // curved meshes
   // error norm: Se (ne x n) | weights | points of QGauss(k+3); curved meshes: x and JxW per point
   DeviceBuffer<double>                  err_tables_;
@@ -134,9 +149,6 @@ std::unique_ptr<LaplaceOperatorBase> create_laplace_operator(exadg_mesh_desc con
                                                              exadg_bc_desc const & bc, int degree,
                                                              int precision, double ip_factor);
 
-// b = RHS of the manufactured solution prod sin(omega x_d) (host, FP64)
-void assemble_rhs_sine(StructuredGrid const & grid, ShapeInfo1D const & shape,
-                       GeometryData const & geo, double omega, std::vector<double> & b);
 
 // dst[i] = 1/d[i] with |d| <= 1e-10 -> 1 (utilities/invert_diagonal.h:35-46)
 template<typename Number>

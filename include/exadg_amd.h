@@ -106,8 +106,33 @@ int exadg_laplace_calculate_diagonal(exadg_laplace * op, void * diag, void * str
 
 /* Poisson::Operator::rhs (operator.cpp:414-423) for the manufactured solution
  * u = prod_d sin(omega x_d): f = 3 omega^2 u, Dirichlet g = u, Neumann h = grad(u).n.
- * b is always FP64. (sine/application.h:32-90 with omega = 3 pi.) */
+ * b is always FP64. (sine/application.h:32-90 with omega = 3 pi.) Assembled on the device;
+ * the first call computes the mesh's right-hand-side point data. */
 int exadg_laplace_rhs_sine(exadg_laplace * op, double omega, double * b, void * stream);
+
+/* Right-hand side for general data. The reference evaluates dealii::Function objects at the
+ * quadrature points (FunctionEvaluator, functions_and_boundary_conditions/evaluate_functions.h:
+ * 151-198); here the caller evaluates its functions at the points this library names and passes
+ * the values.
+ *
+ * Points: the (k+1)^3 Gauss points of every cell (cells lexicographic, points x fastest), and
+ * the (k+1)^2 Gauss points of every boundary face of the non-periodic sides: sides in the order
+ * 0:x- 1:x+ 2:y- 3:y+ 4:z- 5:z+, on a side the faces lexicographic in the two tangential cell
+ * coordinates (lower direction fastest), on a face the points with the lower tangential
+ * direction fastest. */
+int exadg_laplace_rhs_n_points(const exadg_laplace * op, int64_t * n_cell_points,
+                               int64_t * n_boundary_points);
+/* Physical coordinates of those points: FP64 device arrays, component-major (all x, then all y,
+ * then all z), 3 * n_cell_points and 3 * n_boundary_points doubles; either may be NULL. */
+int exadg_laplace_rhs_points(exadg_laplace * op, double * x_cell, double * x_boundary, void * stream);
+/* OperatorBase::rhs (operator_base.cpp:507-554): RHSOperator::evaluate_add (operators/
+ * rhs_operator.cpp:52-95) plus the inhomogeneous boundary terms moved to the right-hand side
+ * (boundary_face_loop_inhom_operator, operator_base.cpp:1436-1491):
+ *   b = int f v + sum_{Dirichlet faces} int (-g dv/dn + 2 tau g v) + sum_{Neumann faces} int h v
+ * with f at the cell points and, at the boundary points, g on Dirichlet sides and h = grad(u).n
+ * on Neumann sides (FP64 device arrays; NULL = zero). b (FP64) is overwritten. */
+int exadg_laplace_rhs(exadg_laplace * op, const double * f, const double * boundary, double * b,
+                      void * stream);
 
 
 /* ---------------------------------------------------------------- preconditioners / solver */
