@@ -577,3 +577,100 @@ class GuessNet:
         check(lib().exadg_guess_predict(self._h, _history(history), _ptr(rhs), _ptr(x0),
                                         x0.numel() // n3, ctypes.byref(fb), _stream(stream)))
         return bool(fb.value)
+
+
+# ---------------------------------------------------------------- snapshots (restart files)
+
+class SnapshotInfo(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("n_vectors", ctypes.c_int32), ("n", ctypes.c_int64), ("degree", ctypes.c_int32),
+                ("mesh", MeshDesc), ("time", ctypes.c_double), ("step", ctypes.c_int64),
+                ("user", ctypes.c_double * 16)]
+
+
+def _snapshot_lib():
+    L = lib()
+    if not getattr(L, "_snapshot_sig", False):
+        vp = ctypes.c_void_p
+        L.exadg_snapshot_write.restype = ctypes.c_int
+        L.exadg_snapshot_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(SnapshotInfo),
+                                           ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, vp]
+        L.exadg_snapshot_read_info.restype = ctypes.c_int
+        L.exadg_snapshot_read_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(SnapshotInfo)]
+        L.exadg_snapshot_read.restype = ctypes.c_int
+        L.exadg_snapshot_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(SnapshotInfo),
+                                          ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int64,
+                                          ctypes.c_int, ctypes.c_int, vp]
+        L._snapshot_sig = True
+    return L
+
+
+def _snapshot_vectors(vectors):
+    """(pointer array, n, precision, on_device) of torch device tensors or numpy host arrays."""
+    import numpy as np
+    if len(vectors) == 0:
+        return (ctypes.c_void_p * 1)(), 0, FP64, 0
+    first = vectors[0]
+    on_device = 0 if isinstance(first, np.ndarray) else 1
+    n = first.size if on_device == 0 else first.numel()
+    if on_device:
+        import torch
+        dtype = first.dtype
+        _require(dtype in (torch.float64, torch.float32), "snapshot vectors must be FP64 or FP32")
+        precision = FP64 if dtype == torch.float64 else FP32
+        for v in vectors:
+            check_vector(v, n, dtype, "snapshot vector")
+        ptrs = [v.data_ptr() for v in vectors]
+    else:
+        dtype = first.dtype
+        _require(dtype in (np.float64, np.float32), "snapshot vectors must be FP64 or FP32")
+        precision = FP64 if dtype == np.float64 else FP32
+        for v in vectors:
+            _require(isinstance(v, np.ndarray) and v.dtype == dtype and v.size == n and
+                     v.flags["C_CONTIGUOUS"], "host snapshot vectors: same size, dtype, contiguous")
+        ptrs = [v.ctypes.data for v in vectors]
+    arr = (ctypes.c_void_p * len(vectors))(*ptrs)
+    return arr, n, precision, on_device
+
+
+def _info_dict(s):
+    return dict(version=s.version, precision=s.precision, n_vectors=s.n_vectors, n=s.n,
+                degree=s.degree, time=s.time, step=s.step, user=list(s.user),
+                mesh=dict(n_cells=list(s.mesh.n_cells), lower=list(s.mesh.lower),
+                          upper=list(s.mesh.upper), deformation=s.mesh.deformation,
+                          frequency=s.mesh.frequency, mapping_degree=s.mesh.mapping_degree,
+                          periodic=list(s.mesh.periodic)))
+
+
+def snapshot_write(path, vectors, time=0.0, step=0, degree=0, mesh=None, user=(), keep=1,
+                   stream=None):
+    """Write vectors (torch device tensors or numpy host arrays, all of one length and dtype) with
+    the discretization parameters, time, step and up to 16 user values; older snapshots are kept
+    as path.old.1 ... up to `keep` files."""
+    arr, n, precision, on_device = _snapshot_vectors(vectors)
+    s = SnapshotInfo()
+    s.precision, s.n_vectors, s.n, s.degree = precision, len(vectors), n, degree
+    if mesh is not None:
+        s.mesh = mesh
+    s.time, s.step = time, step
+    _require(len(user) <= 16, "at most 16 user values")
+    for i, u in enumerate(user):
+        s.user[i] = u
+    check(_snapshot_lib().exadg_snapshot_write(os.fsencode(path), ctypes.byref(s), arr, on_device,
+                                               keep, _stream(stream) if on_device else None))
+
+
+def snapshot_read_info(path):
+    s = SnapshotInfo()
+    check(_snapshot_lib().exadg_snapshot_read_info(os.fsencode(path), ctypes.byref(s)))
+    return _info_dict(s)
+
+
+def snapshot_read(path, vectors, stream=None):
+    """Read a snapshot into the given vectors (their count, length and dtype must match)."""
+    arr, n, precision, on_device = _snapshot_vectors(vectors)
+    s = SnapshotInfo()
+    check(_snapshot_lib().exadg_snapshot_read(os.fsencode(path), ctypes.byref(s), arr, len(vectors),
+                                              n, precision, on_device,
+                                              _stream(stream) if on_device else None))
+    return _info_dict(s)

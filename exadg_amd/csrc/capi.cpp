@@ -85,6 +85,27 @@ ExaDG::Poisson::LaplaceOperator<double> & fp64_operator(exadg_laplace * op)
   return *d;
 }
 
+void check_smooth_arguments(exadg_preconditioner const * p, int64_t n, double const * b,
+                            double const * x)
+{
+  if(p->p == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "the preconditioner has no smoother");
+  if(p->n != n)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "the preconditioner was built for an operator of another size");
+  if(b == x)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "b and x must not alias");
+}
+
+// x += S (b - A x) with the residual in r and the correction in c (device, n_dofs each)
+void smooth_polish(ExaDG::Poisson::LaplaceOperator<double> const & A, ExaDG::PreconditionerBase & P,
+                   double const * b, double * x, double * r, double * c, hipStream_t s)
+{
+  A.residual(r, x, b, s);  // r = b - A x
+  if(!P.smoother_vmult(c, r, s))
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "only the multigrid preconditioner has a smoother");
+  ExaDG::vec::axpby<double>(x, 1.0, c, 1.0, A.m(), s);  // x += S r
+}
+
 template<typename Number>
 void mg_info(ExaDG::PreconditionerBase const * p, int * n_levels, double * lmax, int * degs,
              int * coarse_its)
@@ -309,21 +330,13 @@ int exadg_preconditioner_smooth(exadg_laplace * op, exadg_preconditioner * p, co
 {
   return guarded([&] {
     require(op && p && b && x, "null argument");
-    require(p->p != nullptr, "the preconditioner has no smoother");
-    auto &            A = fp64_operator(op);
-    hipStream_t const s = stream_of(stream);
-    int64_t const     n = A.m();
-    require(p->n == n, "the preconditioner was built for an operator of another size");
-    require(b != x, "b and x must not alias");
+    auto &        A = fp64_operator(op);
+    int64_t const n = A.m();
+    check_smooth_arguments(p, n, b, x);
     p->work_r.reinit(n);
     p->work_c.reinit(n);
-    double * r = p->work_r.data();
-    double * c = p->work_c.data();
-    A.vmult(r, x, s);
-    ExaDG::vec::axpby<double>(r, 1.0, b, -1.0, n, s);  // r = b - A x
-    if(!p->p->smoother_vmult(c, r, s))
-      EXADG_THROW(EXADG_ERR_ARGUMENT, "only the multigrid preconditioner has a smoother");
-    ExaDG::vec::axpby<double>(x, 1.0, c, 1.0, n, s);  // x += S r
+    hipStream_t const s = stream_of(stream);
+    smooth_polish(A, *p->p, b, x, p->work_r.data(), p->work_c.data(), s);
     HIP_CHECK(hipStreamSynchronize(s));
   });
 }
@@ -658,6 +671,44 @@ int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double
     info->tolerance  = si.tolerance;
   }
   return rc != EXADG_OK ? rc : status;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- snapshots
+#include "snapshot.h"
+
+extern "C" {
+
+int exadg_snapshot_write(const char * path, const exadg_snapshot_info * info,
+                         const void * const * vectors, int on_device, int n_snapshots_keep,
+                         void * stream)
+{
+  return guarded([&] {
+    require(path && info, "null argument");
+    ExaDG::write_snapshot(path, *info, vectors, on_device != 0, n_snapshots_keep, stream_of(stream));
+  });
+}
+
+int exadg_snapshot_read_info(const char * path, exadg_snapshot_info * info)
+{
+  return guarded([&] {
+    require(path && info, "null argument");
+    *info = ExaDG::read_snapshot_info(path);
+  });
+}
+
+int exadg_snapshot_read(const char * path, exadg_snapshot_info * info_out, void * const * vectors,
+                        int n_vectors, int64_t n, int precision, int on_device, void * stream)
+{
+  return guarded([&] {
+    require(path != nullptr, "null path");
+    require(n_vectors >= 0 && n >= 0, "negative size");
+    exadg_snapshot_info const info =
+      ExaDG::read_snapshot(path, vectors, n_vectors, n, precision, on_device != 0, stream_of(stream));
+    if(info_out)
+      *info_out = info;
+  });
 }
 
 }  // extern "C"

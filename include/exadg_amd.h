@@ -327,6 +327,40 @@ int exadg_guess_evaluate(exadg_guess * g, exadg_laplace * op, int loss_kind, con
 int exadg_guess_predict(exadg_guess * g, const double * const * history, const double * rhs,
                         double * x0, int64_t n_cells, int * used_fallback, void * stream);
 
+/* ---------------------------------------------------------------- snapshots (restart files) */
+
+/* What a snapshot holds besides its vectors: the parameters the discretization was written with
+ * (DeserializationParameters, time_integration/restart.h:240-300: degree, mapping degree, mesh),
+ * the time and step of the time integrator, and caller data (e.g. the learned guess's scales). */
+typedef struct exadg_snapshot_info
+{
+  int32_t         version;   /* set by the writer */
+  int32_t         precision; /* EXADG_FP64 / EXADG_FP32 of the vectors */
+  int32_t         n_vectors;
+  int64_t         n;         /* entries per vector */
+  int32_t         degree;    /* 0 if the vectors belong to no discretization */
+  exadg_mesh_desc mesh;
+  double          time;
+  int64_t         step;
+  double          user[16];
+} exadg_snapshot_info;
+
+/* Write info->n_vectors vectors of info->n entries (device pointers if on_device, else host
+ * pointers) to `path`. As the reference's restart files (restart.h:48-83), older snapshots are
+ * kept by renaming path -> path.old.1 -> ... up to n_snapshots_keep files in all. The file is
+ * written under a temporary name first and carries a checksum. Synchronizes the stream. */
+int exadg_snapshot_write(const char * path, const exadg_snapshot_info * info,
+                         const void * const * vectors, int on_device, int n_snapshots_keep,
+                         void * stream);
+/* Read a snapshot's header (read_deserialization_parameters, restart.h:270-300). */
+int exadg_snapshot_read_info(const char * path, exadg_snapshot_info * info);
+/* Read a snapshot into n_vectors vectors of n entries of `precision` (device pointers if
+ * on_device). EXADG_ERR_ARGUMENT if the snapshot holds other counts, sizes or precision, or fails
+ * its checksum; the vectors are written only after the whole file is verified. The header is
+ * returned in info_out (may be NULL). Synchronizes the stream. */
+int exadg_snapshot_read(const char * path, exadg_snapshot_info * info_out, void * const * vectors,
+                        int n_vectors, int64_t n, int precision, int on_device, void * stream);
+
 #ifdef __cplusplus
 }
 #endif
