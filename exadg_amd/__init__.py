@@ -674,3 +674,107 @@ def snapshot_read(path, vectors, stream=None):
                                               n, precision, on_device,
                                               _stream(stream) if on_device else None))
     return _info_dict(s)
+
+
+# ---------------------------------------------------------------- pressure step (INS dual splitting)
+
+def _divergence_lib():
+    L = lib()
+    if not getattr(L, "_divergence_sig", False):
+        vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        for name, args in {
+            "exadg_divergence_create": [vp, ctypes.c_int, ctypes.POINTER(vp)],
+            "exadg_divergence_destroy": [vp],
+            "exadg_divergence_n_dofs": [vp, i64p, i64p],
+            "exadg_divergence_apply": [vp, vp, vp, vp],
+            "exadg_divergence_velocity_points": [vp, vp, vp],
+            "exadg_divergence_interpolate_fourier": [vp, dp, ctypes.c_int, ctypes.c_double, vp, vp],
+            "exadg_pressure_step": [vp, vp, vp, vp, ctypes.c_double, ctypes.c_int,
+                                    ctypes.POINTER(vp), vp, vp, ctypes.POINTER(SolverData),
+                                    ctypes.POINTER(SolveInfo), vp],
+        }.items():
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = args
+        L._divergence_sig = True
+    return L
+
+
+class DivergenceOperator:
+    """IncNS::DivergenceOperator (weak form, central flux) from FE_DGQ(degree_u)^3 velocities to
+    the pressure space of a (FP64) LaplaceOperator; velocity layout: per cell three component
+    blocks of (degree_u+1)^3 nodal values."""
+
+    def __init__(self, op, degree_u):
+        self._op = op
+        self._h = ctypes.c_void_p()
+        L = _divergence_lib()
+        check(L.exadg_divergence_create(op.handle, degree_u, ctypes.byref(self._h)))
+        nv, npr = ctypes.c_int64(), ctypes.c_int64()
+        check(L.exadg_divergence_n_dofs(self._h, ctypes.byref(nv), ctypes.byref(npr)))
+        self.degree_u, self.n_velocity, self.n_pressure = degree_u, nv.value, npr.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _divergence_lib().exadg_divergence_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, dst, src, stream=None):
+        import torch
+        check_vector(dst, self.n_pressure, torch.float64, "dst")
+        check_vector(src, self.n_velocity, torch.float64, "src")
+        check(_divergence_lib().exadg_divergence_apply(self._h, _ptr(dst), _ptr(src), _stream(stream)))
+
+    def velocity_points(self, stream=None):
+        """Positions of the velocity nodes, (3, n_velocity / 3) device tensor."""
+        import torch
+        x = torch.empty((3, self.n_velocity // 3), dtype=torch.float64, device="cuda")
+        check(_divergence_lib().exadg_divergence_velocity_points(self._h, _ptr(x), _stream(stream)))
+        return x
+
+    def interpolate_fourier(self, modes, t, u, stream=None):
+        import numpy as np
+        import torch
+        check_vector(u, self.n_velocity, torch.float64, "u")
+        m = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1)
+        _require(m.size % 10 == 0 and m.size > 0, "modes: 10 doubles per mode expected")
+        check(_divergence_lib().exadg_divergence_interpolate_fourier(
+            self._h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m.size // 10, t, _ptr(u),
+            _stream(stream)))
+
+
+def pressure_step(op, prec, div, velocity, scale, history, p_np, rhs, rel_tol=1e-10, abs_tol=1e-20,
+                  max_iter=10000, rel_mode=1, stream=None, raise_on_error=True):
+    """TimeIntBDFDualSplitting::pressure_step: rhs = scale * div(velocity) (scale = -gamma0/dt),
+    p_np = extrapolation of `history` (newest first; [] keeps p_np as the guess), CG solve."""
+    import torch
+    _require(prec is None or prec.n == op.n, "operator and preconditioner sizes differ")
+    _require(len(history) <= 4, "at most 4 history vectors")
+    check_vector(velocity, div.n_velocity, torch.float64, "velocity")
+    check_vector(p_np, op.n, torch.float64, "p_np")
+    check_vector(rhs, op.n, torch.float64, "rhs")
+    for h in history:
+        check_vector(h, op.n, torch.float64, "history vector")
+    d = SolverData(max_iter, abs_tol, rel_tol, rel_mode)
+    info = SolveInfo()
+    rc = _divergence_lib().exadg_pressure_step(
+        op.handle, prec.handle if prec is not None else None, div.handle, _ptr(velocity), scale,
+        len(history), _history(history), _ptr(p_np), _ptr(rhs), ctypes.byref(d), ctypes.byref(info),
+        _stream(stream))
+    out = dict(iterations=info.iterations, converged=bool(info.converged), r0=info.r0,
+               r_final=info.r_final, b_norm=info.b_norm, r_true=info.r_true,
+               tolerance=info.tolerance, status=rc)
+    if rc != EXADG_OK and raise_on_error:
+        raise ExaDGError(rc, lib().exadg_last_error().decode())
+    return out

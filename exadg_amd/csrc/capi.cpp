@@ -8,6 +8,7 @@
 
 #include "../../include/exadg_amd.h"
 #include "common.h"
+#include "divergence_operator.h"
 #include "guess.h"
 #include "krylov.h"
 #include "laplace_operator.h"
@@ -65,6 +66,13 @@ struct exadg_laplace
 struct exadg_guess
 {
   std::unique_ptr<ExaDG::GuessNet> net;
+};
+
+struct exadg_divergence
+{
+  std::unique_ptr<ExaDG::DivergenceOperator> op;
+  ExaDG::Reducer *                           red = nullptr;  // mean of singular right-hand sides
+  ~exadg_divergence() { delete red; }
 };
 
 struct exadg_preconditioner
@@ -482,6 +490,144 @@ int exadg_extrapolate(int order, const double * const * history, double * x0, in
     if(e != hipSuccess)
       EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
   });
+}
+
+int exadg_divergence_create(const exadg_laplace * op, int degree_u, exadg_divergence ** out)
+{
+  return guarded([&] {
+    require(op && out, "null argument");
+    auto h = std::make_unique<exadg_divergence>();
+    h->op  = std::make_unique<ExaDG::DivergenceOperator>(op->op->grid_ptr(), op->op->degree(),
+                                                        degree_u);
+    *out   = h.release();
+  });
+}
+
+int exadg_divergence_destroy(exadg_divergence * div)
+{
+  return guarded([&] { delete div; });
+}
+
+int exadg_divergence_n_dofs(const exadg_divergence * div, int64_t * n_velocity, int64_t * n_pressure)
+{
+  return guarded([&] {
+    require(div && n_velocity && n_pressure, "null argument");
+    *n_velocity = div->op->n_velocity();
+    *n_pressure = div->op->n_pressure();
+  });
+}
+
+int exadg_divergence_apply(exadg_divergence * div, double * dst, const double * src, void * stream)
+{
+  return guarded([&] {
+    require(div, "null operator");
+    div->op->apply(dst, src, stream_of(stream));
+  });
+}
+
+int exadg_divergence_velocity_points(exadg_divergence * div, double * x, void * stream)
+{
+  return guarded([&] {
+    require(div, "null operator");
+    div->op->velocity_points(x, stream_of(stream));
+  });
+}
+
+int exadg_divergence_interpolate_fourier(exadg_divergence * div, const double * modes, int n_modes,
+                                         double t, double * u, void * stream)
+{
+  return guarded([&] {
+    require(div, "null operator");
+    div->op->interpolate_fourier(modes, n_modes, t, u, stream_of(stream));
+  });
+}
+
+int exadg_pressure_step(exadg_laplace * op, exadg_preconditioner * prec, exadg_divergence * div,
+                        const double * velocity, double scale, int order,
+                        const double * const * history, double * p_np, double * rhs,
+                        const exadg_solver_data * data, exadg_solve_info * info, void * stream)
+{
+  int                      status = EXADG_OK;
+  ExaDG::Krylov::SolveInfo si;
+  int const rc = guarded([&] {
+    require(op && div && velocity && p_np && rhs && data, "null argument");
+    require(order >= 0 && order <= 4 && (order == 0 || history != nullptr), "invalid history");
+    auto &            A = fp64_operator(op);
+    hipStream_t const s = stream_of(stream);
+    int64_t const     n = A.m();
+    require(div->op->n_pressure() == n && div->op->degree_p() == A.degree(),
+            "the divergence operator belongs to another pressure space");
+    require(prec == nullptr || prec->n == n,
+            "the preconditioner was built for an operator of another size");
+    require(rhs != p_np && (void const *)rhs != (void const *)velocity, "rhs must not alias");
+    // rhs_pressure (time_int_bdf_dual_splitting.cpp:536-545): rhs = scale * div(u_hat), with
+    // scale = -gamma0 / dt; the pressure level of a singular problem is fixed by its mean
+    div->op->apply(rhs, velocity, s);
+    ExaDG::vec::axpby<double>(rhs, 0.0, rhs, scale, n, s);
+    bool const singular = A.is_singular();
+    if(!div->red)
+      div->red = new ExaDG::Reducer();
+    if(singular)
+      ExaDG::vec::subtract_mean<double>(*div->red, rhs, n, s);
+    // initial guess: extrapolation of the previous pressures (:487-501), or p_np as given
+    if(order > 0)
+    {
+      double beta[4];
+      ExaDG::extrapolation_beta(order, beta);
+      ExaDG::History<double> h;
+      for(int k = 0; k < order; ++k)
+      {
+        require(history[k] != nullptr && history[k] != p_np, "invalid history vector");
+        h.p[k] = history[k];
+      }
+      hipError_t const e =
+        ExaDG::launch_combine(p_np, h, order, beta, nullptr, 0.0, nullptr, n, nullptr, s);
+      if(e != hipSuccess)
+        EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+    }
+    if(!op->cg)
+      op->cg = std::make_unique<ExaDG::Krylov::SolverCG>();
+    ExaDG::Krylov::SolverData sd;
+    sd.max_iter = data->max_iter;
+    sd.abs_tol  = data->abs_tol;
+    sd.rel_tol  = data->rel_tol;
+    sd.rel_mode = data->rel_mode;
+    si          = op->cg->solve(A, prec ? prec->p.get() : nullptr, rhs, p_np, sd, s);
+    // adjust_pressure_level_if_undefined (:515-518): a singular problem keeps zero mean
+    if(singular)
+      ExaDG::vec::subtract_mean<double>(*div->red, p_np, n, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    if(si.not_finite)
+    {
+      g_last_error = "Last iteration step contained NaN or Inf values.";
+      status       = EXADG_ERR_NOT_FINITE;
+    }
+    else if(!si.converged)
+    {
+      g_last_error = "CG reached the iteration limit";
+      status       = EXADG_ERR_NO_CONVERGE;
+    }
+    else if(!si.true_residual_ok)
+    {
+      g_last_error = "CG converged on the recursive residual, but the true residual ||b - A x|| "
+                     "exceeds 10x the tolerance";
+      status       = EXADG_ERR_TRUE_RESIDUAL;
+    }
+  });
+  if(info)
+  {
+    info->iterations = si.iterations;
+    info->converged  = (si.converged && si.true_residual_ok && !si.not_finite) ? 1 : 0;
+    info->not_finite = si.not_finite ? 1 : 0;
+    info->r0         = si.r0;
+    info->r_final    = si.r_final;
+    info->b_norm     = si.b_norm;
+    info->rho        = si.rho;
+    info->n10        = si.n10;
+    info->r_true     = si.r_true;
+    info->tolerance  = si.tolerance;
+  }
+  return rc != EXADG_OK ? rc : status;
 }
 
 int exadg_guess_create(int degree, int history, int width, uint64_t seed, exadg_guess ** out)

@@ -273,6 +273,42 @@ int exadg_vector_random(void * v, int64_t n, int precision, uint64_t seed, void 
 /* v -= mean(v) (FP64, device): the pressure level of a singular problem. */
 int exadg_vector_subtract_mean(double * v, int64_t n, void * stream);
 
+/* ---------------------------------------------------------------- pressure step (INS dual splitting) */
+
+/* The weak DG divergence of a discrete velocity on the pressure space of `op`:
+ * IncNS::DivergenceOperator (incompressible_navier_stokes/spatial_discretization/operators/
+ * divergence_operator.{h,cpp}, :168-298) with the weak formulation, the central flux
+ * (divergence_operator.h:63-68) and boundary data. The velocity lives in FE_DGQ(degree_u)^3 on
+ * the same mesh: per cell three component blocks (x, y, z) of (degree_u+1)^3 nodal values on
+ * Gauss-Lobatto points; QGauss(degree_u+1). Boundaries follow the pressure boundary types of
+ * `op`: a pressure Neumann side is a wall at rest (velocity Dirichlet, u+ = -u-, no flux), a
+ * pressure Dirichlet side an outflow (velocity Neumann, u+ = u-). FP64. The operator shares the
+ * mesh of `op` but not its lifetime. */
+typedef struct exadg_divergence exadg_divergence;
+int exadg_divergence_create(const exadg_laplace * op, int degree_u, exadg_divergence ** out);
+int exadg_divergence_destroy(exadg_divergence * div);
+int exadg_divergence_n_dofs(const exadg_divergence * div, int64_t * n_velocity,
+                            int64_t * n_pressure);
+/* DivergenceOperator::apply (homogeneous operator): dst (pressure) = B src (velocity). */
+int exadg_divergence_apply(exadg_divergence * div, double * dst, const double * src, void * stream);
+/* Positions of the velocity nodes, component-major: 3 * n_cells * (degree_u+1)^3 doubles. */
+int exadg_divergence_velocity_points(exadg_divergence * div, double * x, void * stream);
+/* Nodal interpolation of the Fourier field of exadg_fourier_modes (|n_d| <= 8) at time t into
+ * velocity DoFs u (a synthetic intermediate velocity for pressure sequences). */
+int exadg_divergence_interpolate_fourier(exadg_divergence * div, const double * modes, int n_modes,
+                                         double t, double * u, void * stream);
+/* TimeIntBDFDualSplitting::pressure_step (time_int_bdf_dual_splitting.cpp:478-560) for walls at
+ * rest and no body force: rhs = scale * B velocity (scale = -gamma0 / dt, rhs_pressure :536-545),
+ * mean removed when the pressure problem is singular; p_np = sum_i beta_i history[i]
+ * (ExtrapolationConstants of `order` 1..4, newest first) or, with order 0, p_np as passed
+ * (pressure_last_iter); then the CG solve with `prec` as exadg_cg_solve, and the mean of p_np
+ * removed again for a singular problem (adjust_pressure_level_if_undefined). rhs (n_dofs of op)
+ * returns the right-hand side. Same status codes as exadg_cg_solve. */
+int exadg_pressure_step(exadg_laplace * op, exadg_preconditioner * prec, exadg_divergence * div,
+                        const double * velocity, double scale, int order,
+                        const double * const * history, double * p_np, double * rhs,
+                        const exadg_solver_data * data, exadg_solve_info * info, void * stream);
+
 /* Initial guess by extrapolation, x0 = sum_i beta_i p^{n-i} with the constants of
  * ExtrapolationConstants::set_constant_time_step (extrapolation_constants.cpp:55-93), as used for
  * the pressure at time_int_bdf_dual_splitting.cpp:489-501. history: host array of `order`

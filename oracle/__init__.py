@@ -89,6 +89,7 @@ def lib():
         L.oracle_laplace_n_dofs.argtypes = [ctypes.c_void_p]
         L.oracle_laplace_vmult.argtypes = [ctypes.c_void_p, dp, dp]
         L.oracle_laplace_rhs_sine.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        L.oracle_divergence_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
         L.oracle_laplace_rhs_divergence_fourier.argtypes = [ctypes.c_void_p, dp, ctypes.c_int,
                                                             ctypes.c_double, dp]
         L.oracle_laplace_error_sine.restype = ctypes.c_double
@@ -184,6 +185,15 @@ class LaplaceOperator:
         m = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1)
         b = np.empty(self.n)
         lib().oracle_laplace_rhs_divergence_fourier(self._h, _p(m), m.size // 10, t, _p(b))
+        return b
+
+    def divergence_apply(self, degree_u, u):
+        """Weak DG divergence of the discrete velocity u (per cell three component blocks of
+        (degree_u+1)^3 nodal values), tested with this operator's pressure space."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.size == (self.n // (self.degree + 1) ** 3) * 3 * (degree_u + 1) ** 3
+        b = np.empty(self.n)
+        lib().oracle_divergence_apply(self._h, degree_u, _p(u), _p(b))
         return b
 
     def error_sine(self, omega, u):

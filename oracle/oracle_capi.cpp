@@ -69,6 +69,11 @@ void oracle_laplace_rhs_divergence_fourier(void * h, double const * modes, int n
   static_cast<LaplaceOperator *>(h)->rhs_divergence_fourier(modes, n_modes, t, b);
 }
 
+void oracle_divergence_apply(void * h, int degree_u, double const * u, double * b)
+{
+  divergence_apply(*static_cast<LaplaceOperator *>(h), degree_u, u, b);
+}
+
 double oracle_laplace_error_sine(void * h, double omega, double const * u)
 {
   return static_cast<LaplaceOperator *>(h)->error_sine(omega, u);

@@ -133,6 +133,12 @@ struct LaplaceOperator
   void diagonal(double * d) const;
 };
 
+// b = weak DG divergence of the discrete velocity u (FE_DGQ(degree_u)^3: per cell three
+// component blocks of (degree_u+1)^3 GL nodal values), tested with the pressure space of op,
+// QGauss(degree_u+1), central flux; boundary: pressure Neumann sides are walls at rest (no
+// flux), pressure Dirichlet sides carry u-.n (oracle_divergence.cpp)
+void divergence_apply(LaplaceOperator const & op, int degree_u, double const * u, double * b);
+
 // SplitMix64 counter-based uniform[-1,1] of (seed, index)
 double splitmix_uniform(uint64_t seed, uint64_t index);
 

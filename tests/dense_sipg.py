@@ -217,3 +217,55 @@ def assemble(mesh, degree, ip=1.0, omega=None, plant=None):
                     h = np.einsum("qi,qi->q", grad, nv)
                     rhs[i0:i0 + n3] += (fj * h) @ v
     return A, rhs
+
+
+def assemble_divergence(mesh, degree_p, degree_u):
+    """Dense weak DG divergence B (pressure x velocity) of the pressure step
+    (divergence_operator.cpp:168-298, weak form, central flux):
+      (B u)_i = -sum_K int u . grad q_i + sum_F int {u}.n q_i
+    with QGauss(degree_u + 1) on cells and faces. Velocity DoFs per cell: three component blocks
+    of (degree_u+1)^3 nodal values. Boundary: where the pressure side is Neumann the velocity is
+    Dirichlet at rest (u+ = -u-, no flux); where it is Dirichlet the velocity is Neumann
+    (u+ = u-, flux u-.n)."""
+    npp, nu = degree_p + 1, degree_u + 1
+    n3p, n3u = npp ** 3, nu ** 3
+    bp, bu = Basis1D(gl_nodes(npp)), Basis1D(gl_nodes(nu))
+    gp, gw = gauss(nu)
+    ncell = len(mesh.cells)
+    B = np.zeros((ncell * n3p, ncell * 3 * n3u))
+    qx = np.array([[gp[i], gp[j], gp[k]] for k in range(nu) for j in range(nu) for i in range(nu)])
+    qw = np.array([gw[i] * gw[j] * gw[k] for k in range(nu) for j in range(nu) for i in range(nu)])
+    vu, _ = _cell_basis(bu, qx)
+    _, gpref = _cell_basis(bp, qx)
+    for cc in mesh.cells:
+        _, J = mesh.map(cc, qx)
+        Ji = np.linalg.inv(J)
+        jxw = np.linalg.det(J) * qw
+        gphys = np.einsum("qji,qmj->qmi", Ji, gpref)  # grad q_m at the points
+        ip, iu = mesh.index(cc) * n3p, mesh.index(cc) * 3 * n3u
+        for k in range(3):
+            B[ip:ip + n3p, iu + k * n3u:iu + (k + 1) * n3u] -= np.einsum("q,qm,ql->ml", jxw,
+                                                                         gphys[:, :, k], vu)
+    for cc in mesh.cells:
+        for d in range(3):
+            for s in range(2):
+                nb = mesh.neighbor(cc, d, s)
+                xi, _, _, fj, nv = _face_geo(mesh, cc, d, s, gp, gw)
+                qm = _cell_basis(bp, xi)[0]  # own pressure test at the face points
+                um = _cell_basis(bu, xi)[0]
+                ip, iu = mesh.index(cc) * n3p, mesh.index(cc) * 3 * n3u
+                if nb is not None:
+                    xin = xi.copy()
+                    xin[:, d] = 1.0 - s  # the neighbour's side of the shared face
+                    up = _cell_basis(bu, xin)[0]
+                    jn = mesh.index(nb) * 3 * n3u
+                    for k in range(3):
+                        B[ip:ip + n3p, iu + k * n3u:iu + (k + 1) * n3u] += \
+                            0.5 * np.einsum("q,qm,ql->ml", fj * nv[:, k], qm, um)
+                        B[ip:ip + n3p, jn + k * n3u:jn + (k + 1) * n3u] += \
+                            0.5 * np.einsum("q,qm,ql->ml", fj * nv[:, k], qm, up)
+                elif mesh.bc[2 * d + s] == 0:  # pressure Dirichlet: velocity Neumann
+                    for k in range(3):
+                        B[ip:ip + n3p, iu + k * n3u:iu + (k + 1) * n3u] += \
+                            np.einsum("q,qm,ql->ml", fj * nv[:, k], qm, um)
+    return B
