@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libexadg_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "exadg_amd.h")
 
 EXADG_OK = 0
+EXADG_ERR_ARGUMENT, EXADG_ERR_HIP, EXADG_ERR_NOT_FINITE, EXADG_ERR_NO_CONVERGE = 1, 2, 3, 4
 FP64, FP32 = 0, 1
 DIRICHLET, NEUMANN = 0, 1
 GUESS_LOSS_L2, GUESS_LOSS_ENERGY, GUESS_LOSS_RESIDUAL = 0, 1, 2
@@ -154,6 +155,8 @@ def lib():
             "exadg_preconditioner_mg_info": [vp, ip, ctypes.POINTER(ctypes.c_double), ip, ip],
             "exadg_cg_solve": [vp, vp, vp, vp, ctypes.POINTER(SolverData), ctypes.POINTER(SolveInfo),
                                vp],
+            "exadg_cg_solve_noalloc": [vp, vp, vp, vp, ctypes.POINTER(SolverData),
+                                       ctypes.POINTER(SolveInfo), vp],
             "exadg_laplace_error_sine": [vp, ctypes.c_double, vp, dp, vp],
             "exadg_fourier_modes": [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ip],
             "exadg_laplace_rhs_divergence_fourier": [vp, dp, ctypes.c_int, ctypes.c_double, vp, vp],
@@ -402,18 +405,24 @@ class Preconditioner:
                     last_coarse_iterations=cits.value)
 
 
+NOALLOC_MAX_DOFS = 1 << 22  # EXADG_NOALLOC_MAX_DOFS
+
+
 def cg_solve(op, b, x, prec=None, rel_tol=1e-10, abs_tol=1e-20, max_iter=10000, rel_mode=0,
-             stream=None, raise_on_error=True):
+             stream=None, raise_on_error=True, noalloc=False):
     """Krylov::KrylovSolver::solve (CG) on the FP64 operator; x holds the initial guess.
-    The result carries the final FP64 true residual r_true = ||b - A x||."""
+    The result carries the final FP64 true residual r_true = ||b - A x||. noalloc=True calls
+    exadg_cg_solve_noalloc (work vectors in the library's static device workspace; at most
+    NOALLOC_MAX_DOFS DoF)."""
     import torch
     _require(prec is None or prec.n == op.n, "operator and preconditioner sizes differ")
     check_vector(b, op.n, torch.float64, "b")
     check_vector(x, op.n, torch.float64, "x")
     d = SolverData(max_iter, abs_tol, rel_tol, rel_mode)
     info = SolveInfo()
-    rc = lib().exadg_cg_solve(op.handle, prec.handle if prec is not None else None, _ptr(b), _ptr(x),
-                              ctypes.byref(d), ctypes.byref(info), _stream(stream))
+    fn = lib().exadg_cg_solve_noalloc if noalloc else lib().exadg_cg_solve
+    rc = fn(op.handle, prec.handle if prec is not None else None, _ptr(b), _ptr(x),
+            ctypes.byref(d), ctypes.byref(info), _stream(stream))
     out = dict(iterations=info.iterations, converged=bool(info.converged),
                not_finite=bool(info.not_finite), r0=info.r0, r_final=info.r_final,
                b_norm=info.b_norm, rho=info.rho, n10=info.n10, r_true=info.r_true,

@@ -13,6 +13,7 @@
 #include "krylov.h"
 #include "laplace_operator.h"
 #include "multigrid.h"
+#include "noalloc.h"
 
 namespace
 {
@@ -112,6 +113,81 @@ void smooth_polish(ExaDG::Poisson::LaplaceOperator<double> const & A, ExaDG::Pre
   if(!P.smoother_vmult(c, r, s))
     EXADG_THROW(EXADG_ERR_ARGUMENT, "only the multigrid preconditioner has a smoother");
   ExaDG::vec::axpby<double>(x, 1.0, c, 1.0, A.m(), s);  // x += S r
+}
+
+// exadg_cg_solve and its no-allocation variant: the same CG loop, on the handle's own work
+// vectors or on the static device workspace
+int cg_solve_entry(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
+                   const exadg_solver_data * data, exadg_solve_info * info, void * stream,
+                   bool noalloc)
+{
+  int                      status = EXADG_OK;
+  ExaDG::Krylov::SolveInfo si;
+  int const rc = guarded([&] {
+    require(op && b && x && data, "null argument");
+    require(data->max_iter >= 0 && data->rel_tol >= 0 && data->abs_tol >= 0, "invalid solver data");
+    auto & A = fp64_operator(op);
+    require(prec == nullptr || prec->n == A.m(),
+            "the preconditioner was built for an operator of another size");
+    ExaDG::Krylov::SolverData sd;
+    sd.max_iter = data->max_iter;
+    sd.abs_tol  = data->abs_tol;
+    sd.rel_tol  = data->rel_tol;
+    sd.rel_mode = data->rel_mode;
+    hipStream_t const               s = stream_of(stream);
+    ExaDG::PreconditionerBase * const P = prec ? prec->p.get() : nullptr;
+    if(noalloc)
+    {
+      require(A.m() <= ExaDG::kNoAllocMaxDofs,
+              "the operator has more DoFs than the no-allocation workspace holds "
+              "(EXADG_NOALLOC_MAX_DOFS)");
+      require(P == nullptr || !P->vmult_allocates(),
+              "a multigrid preconditioner that replays HIP graphs captures them on first use; "
+              "create it with capture_graph = 0 for the no-allocation solve");
+      require(b != x, "b and x must not alias");
+      int64_t const        N = ExaDG::kNoAllocMaxDofs;
+      double * const       w = ExaDG::noalloc_workspace();
+      ExaDG::Reducer       red(w + 4 * N);
+      red.clear_counter(s);
+      si = ExaDG::Krylov::cg_solve_in(A, P, b, x, sd, w, w + N, w + 2 * N, w + 3 * N, red, s);
+    }
+    else
+    {
+      if(!op->cg)
+        op->cg = std::make_unique<ExaDG::Krylov::SolverCG>();
+      si = op->cg->solve(A, P, b, x, sd, s);
+    }
+    if(si.not_finite)
+    {
+      g_last_error = "Last iteration step contained NaN or Inf values.";
+      status       = EXADG_ERR_NOT_FINITE;
+    }
+    else if(!si.converged)
+    {
+      g_last_error = "CG reached the iteration limit";
+      status       = EXADG_ERR_NO_CONVERGE;
+    }
+    else if(!si.true_residual_ok)
+    {
+      g_last_error = "CG converged on the recursive residual, but the true residual ||b - A x|| "
+                     "exceeds 10x the tolerance";
+      status       = EXADG_ERR_TRUE_RESIDUAL;
+    }
+  });
+  if(info)
+  {
+    info->iterations = si.iterations;
+    info->converged  = (si.converged && si.true_residual_ok && !si.not_finite) ? 1 : 0;
+    info->not_finite = si.not_finite ? 1 : 0;
+    info->r0         = si.r0;
+    info->r_final    = si.r_final;
+    info->b_norm     = si.b_norm;
+    info->rho        = si.rho;
+    info->n10        = si.n10;
+    info->r_true     = si.r_true;
+    info->tolerance  = si.tolerance;
+  }
+  return rc != EXADG_OK ? rc : status;
 }
 
 template<typename Number>
@@ -770,53 +846,14 @@ int exadg_guess_predict(exadg_guess * g, const double * const * history, const d
 int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
                    const exadg_solver_data * data, exadg_solve_info * info, void * stream)
 {
-  int                           status = EXADG_OK;
-  ExaDG::Krylov::SolveInfo      si;
-  int const rc = guarded([&] {
-    require(op && b && x && data, "null argument");
-    require(data->max_iter >= 0 && data->rel_tol >= 0 && data->abs_tol >= 0, "invalid solver data");
-    auto & A = fp64_operator(op);
-    require(prec == nullptr || prec->n == A.m(),
-            "the preconditioner was built for an operator of another size");
-    if(!op->cg)
-      op->cg = std::make_unique<ExaDG::Krylov::SolverCG>();
-    ExaDG::Krylov::SolverData sd;
-    sd.max_iter = data->max_iter;
-    sd.abs_tol  = data->abs_tol;
-    sd.rel_tol  = data->rel_tol;
-    sd.rel_mode = data->rel_mode;
-    si          = op->cg->solve(A, prec ? prec->p.get() : nullptr, b, x, sd, stream_of(stream));
-    if(si.not_finite)
-    {
-      g_last_error = "Last iteration step contained NaN or Inf values.";
-      status       = EXADG_ERR_NOT_FINITE;
-    }
-    else if(!si.converged)
-    {
-      g_last_error = "CG reached the iteration limit";
-      status       = EXADG_ERR_NO_CONVERGE;
-    }
-    else if(!si.true_residual_ok)
-    {
-      g_last_error = "CG converged on the recursive residual, but the true residual ||b - A x|| "
-                     "exceeds 10x the tolerance";
-      status       = EXADG_ERR_TRUE_RESIDUAL;
-    }
-  });
-  if(info)
-  {
-    info->iterations = si.iterations;
-    info->converged  = (si.converged && si.true_residual_ok && !si.not_finite) ? 1 : 0;
-    info->not_finite = si.not_finite ? 1 : 0;
-    info->r0         = si.r0;
-    info->r_final    = si.r_final;
-    info->b_norm     = si.b_norm;
-    info->rho        = si.rho;
-    info->n10        = si.n10;
-    info->r_true     = si.r_true;
-    info->tolerance  = si.tolerance;
-  }
-  return rc != EXADG_OK ? rc : status;
+  return cg_solve_entry(op, prec, b, x, data, info, stream, false);
+}
+
+int exadg_cg_solve_noalloc(exadg_laplace * op, exadg_preconditioner * prec, const double * b,
+                           double * x, const exadg_solver_data * data, exadg_solve_info * info,
+                           void * stream)
+{
+  return cg_solve_entry(op, prec, b, x, data, info, stream, true);
 }
 
 }  // extern "C"

@@ -13,19 +13,31 @@ SolveInfo SolverCG::solve(Poisson::LaplaceOperator<double> const & A, Preconditi
     EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
   if(b == x)
     EXADG_THROW(EXADG_ERR_ARGUMENT, "b and x must not alias");
+  ws_.reinit(A.m());
+  return cg_solve_in(A, P, b, x, data, ws_.r.data(), ws_.z.data(), ws_.p.data(), ws_.v.data(), red_,
+                     s);
+}
+
+SolveInfo cg_solve_in(Poisson::LaplaceOperator<double> const & A, PreconditionerBase * P,
+                      double const * b, double * x, SolverData const & data, double * r, double * z,
+                      double * p, double * v, Reducer & red, hipStream_t s)
+{
+  if(b == nullptr || x == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  if(b == x)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "b and x must not alias");
   int64_t const n = A.m();
-  ws_.reinit(n);
-  double *  r = ws_.r.data(), *z = P ? ws_.z.data() : ws_.r.data(), *p = ws_.p.data(),
-         *  v = ws_.v.data();
+  if(!P)
+    z = r;
   SolveInfo info;
 
   // r = b - A x, z = P r
   A.residual(r, x, b, s);
   if(P)
     P->vmult(z, r, s);
-  vec::norm2_and_dot(red_, 1, 2, r, z, n, s);
-  vec::dot(red_, 4, b, b, n, s);
-  double const * h = red_.fetch(s, 5);
+  vec::norm2_and_dot(red, 1, 2, r, z, n, s);
+  vec::dot(red, 4, b, b, n, s);
+  double const * h = red.fetch(s, 5);
   info.r0          = std::sqrt(h[1]);
   info.b_norm      = std::sqrt(h[4]);
   info.r_final     = info.r0;
@@ -47,12 +59,12 @@ SolveInfo SolverCG::solve(Poisson::LaplaceOperator<double> const & A, Preconditi
   {
     int const old_slot = 2 + (it - 1) % 2, new_slot = 2 + it % 2;
     A.vmult(v, p, s);
-    vec::dot(red_, 0, p, v, n, s);
-    vec::cg_update(x, r, p, v, red_.result() + old_slot, red_.result(), n, s);
+    vec::dot(red, 0, p, v, n, s);
+    vec::cg_update(x, r, p, v, red.result() + old_slot, red.result(), n, s);
     if(P)
       P->vmult(z, r, s);
-    vec::norm2_and_dot(red_, 1, new_slot, r, z, n, s);
-    h               = red_.fetch(s, 4);
+    vec::norm2_and_dot(red, 1, new_slot, r, z, n, s);
+    h               = red.fetch(s, 4);
     double const rn = std::sqrt(h[1]);
     info.iterations = it;
     info.r_final    = rn;
@@ -66,14 +78,14 @@ SolveInfo SolverCG::solve(Poisson::LaplaceOperator<double> const & A, Preconditi
       info.converged = true;
       break;
     }
-    vec::cg_direction(p, z, red_.result() + new_slot, red_.result() + old_slot, n, s);
+    vec::cg_direction(p, z, red.result() + new_slot, red.result() + old_slot, n, s);
   }
   if(!info.not_finite)
   {
     // final true residual in FP64: one fused residual kernel and a norm
     A.residual(v, x, b, s);
-    vec::dot(red_, 5, v, v, n, s);
-    info.r_true = std::sqrt(red_.fetch(s, 6)[5]);
+    vec::dot(red, 5, v, v, n, s);
+    info.r_true = std::sqrt(red.fetch(s, 6)[5]);
     if(!std::isfinite(info.r_true))
       info.not_finite = true;
     else

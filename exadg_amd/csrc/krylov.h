@@ -33,6 +33,12 @@ struct SolveInfo
   double rho = 0, n10 = 0;
 };
 
+// the CG loop on caller-provided work vectors r, z, p, v (length A.m(); z unused without a
+// preconditioner) and reduction workspace; allocates nothing itself
+SolveInfo cg_solve_in(Poisson::LaplaceOperator<double> const & A, PreconditionerBase * P,
+                      double const * b, double * x, SolverData const & data, double * r, double * z,
+                      double * p, double * v, Reducer & red, hipStream_t s);
+
 class SolverCG
 {
 public:

@@ -32,6 +32,8 @@ public:
   {
     return false;
   }
+  // true if vmult may allocate device memory or HIP objects (graph capture on first use)
+  virtual bool vmult_allocates() const { return false; }
 };
 
 // point Jacobi (jacobi_preconditioner.h:33-82)
@@ -177,6 +179,7 @@ public:
   void vmult(double * dst, double const * src, hipStream_t s) override;
   void update(hipStream_t s) override;
   bool smoother_vmult(double * dst, double const * src, hipStream_t s) override;
+  bool vmult_allocates() const override { return uses_graph(); }
 
   int n_levels() const { return (int)levels_.size(); }
   // level l: 0 = coarsest

@@ -263,22 +263,41 @@ __global__ void __launch_bounds__(kBlock) invert_diagonal_kernel(Number * d, int
 
 Reducer::Reducer()
 {
-  partials_.resize(2 * kMaxPartials);
-  result_.resize(kSlots);
-  counter_.resize(1);
-  HIP_CHECK(hipMemset(counter_.data(), 0, sizeof(unsigned int)));
+  own_.resize(kDeviceDoubles);
+  partials_ = own_.data();
+  result_   = partials_ + 2 * kMaxPartials;
+  counter_  = reinterpret_cast<unsigned int *>(result_ + kSlots);
+  HIP_CHECK(hipMemset(counter_, 0, sizeof(unsigned int)));
   HIP_CHECK(hipHostMalloc((void **)&host_, kSlots * sizeof(double), hipHostMallocDefault));
+  pinned_ = true;
+}
+
+Reducer::Reducer(double * device_memory)
+{
+  if(device_memory == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null reduction workspace");
+  partials_ = device_memory;
+  result_   = partials_ + 2 * kMaxPartials;
+  counter_  = reinterpret_cast<unsigned int *>(result_ + kSlots);
+  host_     = host_local_;
 }
 
 Reducer::~Reducer()
 {
-  if(host_)
+  if(pinned_ && host_)
     (void)hipHostFree(host_);
+}
+
+void Reducer::clear_counter(hipStream_t s)
+{
+  hipLaunchKernelGGL((set_zero_kernel<double>), dim3(1), dim3(kBlock), 0, s,
+                     reinterpret_cast<double *>(counter_), int64_t(1));
+  HIP_CHECK(hipGetLastError());
 }
 
 double const * Reducer::fetch(hipStream_t s, int n_slots)
 {
-  HIP_CHECK(hipMemcpyAsync(host_, result_.data(), n_slots * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(host_, result_, n_slots * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   return host_;
 }

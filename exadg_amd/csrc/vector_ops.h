@@ -30,7 +30,7 @@ private:
 };
 
 // Reduction workspace: partial sums per block, the final scalars and the ticket counter of the
-// one-launch reduction (device), plus a pinned host mirror for the values the host needs
+// one-launch reduction (device), plus a host mirror for the values the host needs
 // (convergence checks). Reductions on one Reducer must be stream-ordered (they share the
 // partials and the counter).
 class Reducer
@@ -38,20 +38,34 @@ class Reducer
 public:
   static constexpr int kMaxPartials = 1024;
   static constexpr int kSlots       = 8;
+  // device doubles a Reducer needs: 2 kMaxPartials partials, kSlots results, one word for
+  // the counter
+  static constexpr int kDeviceDoubles = 2 * kMaxPartials + kSlots + 1;
+  // owning: allocates the device buffers and a pinned host mirror
   Reducer();
+  // non-owning view over kDeviceDoubles doubles of caller device memory (256-byte aligned);
+  // allocates nothing. The counter word must be zero before the first reduction
+  // (clear_counter). fetch() then copies through pageable memory held by the object.
+  explicit Reducer(double * device_memory);
   ~Reducer();
   Reducer(Reducer const &) = delete;
   Reducer & operator=(Reducer const &) = delete;
-  double * partials() { return partials_.data(); }
-  double * result() { return result_.data(); }  // device, kSlots doubles
-  unsigned int * counter() { return counter_.data(); }  // device, zero between reductions
+  double * partials() { return partials_; }
+  double * result() { return result_; }  // device, kSlots doubles
+  unsigned int * counter() { return counter_; }  // device, zero between reductions
+  // zero the counter word with a fill kernel (graph-capturable; no memset node)
+  void clear_counter(hipStream_t s);
   // copy the result slots to the host and wait
   double const * fetch(hipStream_t s, int n_slots = kSlots);
 
 private:
-  DeviceBuffer<double>       partials_, result_;
-  DeviceBuffer<unsigned int> counter_;
-  double *             host_ = nullptr;
+  DeviceBuffer<double> own_;
+  double *             partials_ = nullptr;
+  double *             result_   = nullptr;
+  unsigned int *       counter_  = nullptr;
+  double *             host_     = nullptr;  // pinned (owning mode) or host_local_
+  bool                 pinned_   = false;
+  double               host_local_[kSlots] = {};
 };
 
 namespace vec

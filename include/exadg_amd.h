@@ -239,6 +239,24 @@ typedef struct exadg_solve_info
 int exadg_cg_solve(exadg_laplace * op, exadg_preconditioner * prec, const double * b, double * x,
                    const exadg_solver_data * data, exadg_solve_info * info, void * stream);
 
+/* Largest operator (DoF) that exadg_cg_solve_noalloc accepts. */
+#define EXADG_NOALLOC_MAX_DOFS (INT64_C(1) << 22)
+
+/* No counterpart in the reference: exadg_cg_solve for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc). Same arguments, results and
+ * status codes as exadg_cg_solve, and the same kernels in the same order (so the same x), but
+ * the CG work vectors and the reduction buffers live in a static device array that each GPU
+ * holds from library load until the process ends (4 * EXADG_NOALLOC_MAX_DOFS + 2057 doubles,
+ * about 128 MiB); the call allocates nothing. Rejected with EXADG_ERR_ARGUMENT: an operator
+ * with more than EXADG_NOALLOC_MAX_DOFS DoF, and a multigrid preconditioner created with
+ * capture_graph (its first replay captures a graph). The preconditioner and operator must have
+ * been created (and updated) beforehand. Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (two streams, two threads) race and must be serialized by the
+ * caller. */
+int exadg_cg_solve_noalloc(exadg_laplace * op, exadg_preconditioner * prec, const double * b,
+                           double * x, const exadg_solver_data * data, exadg_solve_info * info,
+                           void * stream);
+
 /* Relative L2 error of u (FP64, device) against u* = prod_d sin(omega x_d) with QGauss(k+3)
  * on the mapped cells: the number the reference's sine application prints
  * (ErrorCalculator::calculate_error, postprocessor/error_calculation.cpp:38-103). FP64
