@@ -110,13 +110,17 @@ def lib():
         L.oracle_mg_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(ctypes.c_int),
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_double]
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.oracle_mg_level_list.restype = ctypes.c_int
+        L.oracle_mg_level_list.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int] + \
+            [ctypes.POINTER(ctypes.c_int)] * 3 + [ctypes.c_int]
         L.oracle_mg_destroy.argtypes = [ctypes.c_void_p]
         L.oracle_mg_vmult.argtypes = [ctypes.c_void_p, dp, dp]
         L.oracle_mg_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int),
-                                     ctypes.POINTER(ctypes.c_int)]
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.oracle_mg_level_vmult.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
         L.oracle_mg_prolongate_add.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
         L.oracle_mg_restrict_add.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
@@ -245,16 +249,35 @@ def bisect_degrees(k):
     return out
 
 
+# MultigridType (multigrid_parameters.h:39-57), numbered as there; 0 = pcMG/pMG by coarse_q1
+MG_TYPES = ["legacy", "hMG", "chMG", "hcMG", "cMG", "pMG", "cpMG", "pcMG", "hpMG", "chpMG",
+            "hcpMG", "hpcMG", "phMG", "cphMG", "pchMG", "phcMG"]
+
+
+def mg_level_list(mg_type, degrees, n_h_levels=1, coarse_q1=True):
+    """Levels (coarse to fine) of a multigrid type as (h, degree, dg) tuples; mg_type is a name
+    of MG_TYPES or its index."""
+    t = MG_TYPES.index(mg_type) if isinstance(mg_type, str) else int(mg_type)
+    arr = (ctypes.c_int * len(degrees))(*degrees)
+    h, deg, dg = (ctypes.c_int * 64)(), (ctypes.c_int * 64)(), (ctypes.c_int * 64)()
+    n = lib().oracle_mg_level_list(t, arr, len(degrees), n_h_levels, int(coarse_q1), h, deg, dg, 64)
+    if n < 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return [(h[i], deg[i], bool(dg[i])) for i in range(n)]
+
+
 class Multigrid:
-    """Oracle hybrid p-multigrid V-cycle (FP64)."""
+    """Oracle hybrid multigrid V-cycle (FP64) over h-, p- and c-levels."""
 
     def __init__(self, mesh, degrees, coarse_q1=True, smoother_degree=5, smoothing_range=20.0,
                  eig_its=20, coarse_solver=0, coarse_rel_tol=1e-3, coarse_max_iter=10000,
-                 singular=False, ip_factor=1.0):
+                 singular=False, ip_factor=1.0, mg_type=0, n_h_levels=1):
+        t = MG_TYPES.index(mg_type) if isinstance(mg_type, str) else int(mg_type)
         arr = (ctypes.c_int * len(degrees))(*degrees)
         self._h = lib().oracle_mg_create(ctypes.byref(mesh), arr, len(degrees), int(coarse_q1),
                                          smoother_degree, smoothing_range, eig_its, coarse_solver,
-                                         coarse_rel_tol, coarse_max_iter, int(singular), ip_factor)
+                                         coarse_rel_tol, coarse_max_iter, int(singular), ip_factor,
+                                         t, n_h_levels)
         if not self._h:
             raise RuntimeError(lib().oracle_last_error().decode())
         self.info()
@@ -268,10 +291,13 @@ class Multigrid:
         nl = ctypes.c_int()
         n = (ctypes.c_longlong * 32)()
         deg, dg, cd = (ctypes.c_int * 32)(), (ctypes.c_int * 32)(), (ctypes.c_int * 32)()
+        hl = (ctypes.c_int * 32)()
         lmax = np.zeros(32)
         cits = ctypes.c_int()
-        lib().oracle_mg_info(self._h, ctypes.byref(nl), n, deg, dg, _p(lmax), cd, ctypes.byref(cits))
+        lib().oracle_mg_info(self._h, ctypes.byref(nl), n, deg, dg, _p(lmax), cd, ctypes.byref(cits),
+                             hl)
         L = nl.value
+        self.h_levels = [hl[i] for i in range(L)]
         self.n_levels = L
         self.sizes = [n[i] for i in range(L)]
         self.degrees = [deg[i] for i in range(L)]

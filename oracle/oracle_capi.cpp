@@ -146,11 +146,11 @@ int oracle_cg_solve(void * h, int use_jacobi, double const * b, double * x, doub
   }
 }
 
-// hybrid p-multigrid (DG degrees fine to coarse, optional continuous Q1 level)
+// hybrid multigrid (p-sequence: degrees fine to coarse; type: MGType; n_h_levels meshes)
 void * oracle_mg_create(oracle_mesh const * m, int const * degrees, int n_degrees, int coarse_q1,
                         int smoother_degree, double smoothing_range, int eig_its,
                         int coarse_solver, double coarse_rel_tol, int coarse_max_iter,
-                        int singular, double ip_factor)
+                        int singular, double ip_factor, int type, int n_h_levels)
 {
   try
   {
@@ -159,6 +159,8 @@ void * oracle_mg_create(oracle_mesh const * m, int const * degrees, int n_degree
     MGDesc md;
     md.degrees.assign(degrees, degrees + n_degrees);
     md.coarse_q1       = coarse_q1 != 0;
+    md.type            = type;
+    md.n_h_levels      = n_h_levels;
     md.smoother_degree = smoother_degree;
     md.smoothing_range = smoothing_range;
     md.eig_its         = eig_its;
@@ -182,9 +184,33 @@ void oracle_mg_vmult(void * h, double * dst, double const * src)
   static_cast<Multigrid *>(h)->vmult(dst, src);
 }
 
-// per level (0 = coarsest): n, degree, dg flag, lambda_max estimate, Chebyshev degree
+// level list of a multigrid type (coarse to fine): h, degree, dg per level; returns the count
+// (-1 and the message on an invalid combination)
+int oracle_mg_level_list(int type, int const * degrees, int n_degrees, int n_h_levels,
+                         int coarse_q1, int * h, int * degree, int * dg, int max_levels)
+{
+  try
+  {
+    std::vector<LevelInfo> const L =
+      mg_level_list(type, std::vector<int>(degrees, degrees + n_degrees), n_h_levels, coarse_q1 != 0);
+    for(size_t l = 0; l < L.size() && (int)l < max_levels; ++l)
+    {
+      h[l]      = L[l].h;
+      degree[l] = L[l].degree;
+      dg[l]     = L[l].dg ? 1 : 0;
+    }
+    return (int)L.size();
+  }
+  catch(std::exception const & e)
+  {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// per level (0 = coarsest): n, degree, dg flag, lambda_max estimate, Chebyshev degree, h
 void oracle_mg_info(void * h, int * n_levels, long long * n, int * degree, int * dg, double * lmax,
-                    int * cheb_degree, int * coarse_its)
+                    int * cheb_degree, int * coarse_its, int * h_level)
 {
   auto * mg = static_cast<Multigrid *>(h);
   *n_levels = (int)mg->levels.size();
@@ -192,6 +218,7 @@ void oracle_mg_info(void * h, int * n_levels, long long * n, int * degree, int *
   {
     auto const & L   = mg->levels[l];
     n[l]             = L.n;
+    h_level[l]       = L.h;
     degree[l]        = L.degree;
     dg[l]            = L.dg ? 1 : 0;
     Chebyshev const & c = l == 0 ? mg->coarse_cheb : L.smoother;

@@ -249,6 +249,107 @@ void Q1Operator::diagonal(double * dg) const
       dg[i] = 1.0;
 }
 
+// ---------------------------------------------------------------- continuous Q_p levels
+
+CGSpace::CGSpace(MeshDesc const & md, int p_) : d(md), p(p_)
+{
+  for(int k = 0; k < 3; ++k)
+    nn[k] = d.n_cells[k] * p + (d.periodic[k] ? 0 : 1);
+  n_nodes = (int64_t)nn[0] * nn[1] * nn[2];
+  constrained.assign(n_nodes, 0);
+  for(int64_t g = 0; g < n_nodes; ++g)
+  {
+    int const c[3] = {(int)(g % nn[0]), (int)((g / nn[0]) % nn[1]), (int)(g / ((int64_t)nn[0] * nn[1]))};
+    for(int k = 0; k < 3; ++k)
+      if(!d.periodic[k] && ((c[k] == 0 && d.boundary_type[2 * k] == 0) ||
+                            (c[k] == nn[k] - 1 && d.boundary_type[2 * k + 1] == 0)))
+        constrained[g] = 1;
+  }
+}
+
+int64_t CGSpace::node(int64_t c, int local) const
+{
+  int const n = p + 1;
+  int64_t   cc[3];
+  cc[0] = c % d.n_cells[0];
+  cc[1] = (c / d.n_cells[0]) % d.n_cells[1];
+  cc[2] = c / ((int64_t)d.n_cells[0] * d.n_cells[1]);
+  int const loc[3] = {local % n, (local / n) % n, local / (n * n)};
+  int64_t   g[3];
+  for(int k = 0; k < 3; ++k)
+  {
+    g[k] = cc[k] * p + loc[k];
+    if(d.periodic[k] && g[k] == nn[k])
+      g[k] = 0;
+  }
+  return g[0] + (int64_t)nn[0] * (g[1] + (int64_t)nn[1] * g[2]);
+}
+
+int CGSpace::valence(int64_t g) const
+{
+  int64_t const c[3] = {g % nn[0], (g / nn[0]) % nn[1], g / ((int64_t)nn[0] * nn[1])};
+  int           v    = 1;
+  for(int k = 0; k < 3; ++k)
+  {
+    bool const vertex = c[k] % p == 0;
+    bool const end    = !d.periodic[k] && (c[k] == 0 || c[k] == nn[k] - 1);
+    if(vertex && !end)
+      v *= 2;
+  }
+  return v;
+}
+
+CGOperator::CGOperator(MeshDesc const & d, int p, double ip)
+  : space(d, p), cells(std::make_unique<LaplaceOperator>(d, p, ip))
+{
+}
+
+void CGOperator::vmult(double * dst, double const * src) const
+{
+  // OperatorBase on a continuous DoFHandler: gather (constrained entries zero), cell integrals,
+  // assembly; constrained rows act as the identity (operator_base.cpp:299-309)
+  int64_t const       nc = space.d.n_cells[0] * (int64_t)space.d.n_cells[1] * space.d.n_cells[2];
+  int const           n3 = (space.p + 1) * (space.p + 1) * (space.p + 1);
+  std::vector<double> u((size_t)nc * n3), v((size_t)nc * n3, 0.0);
+  for(int64_t c = 0; c < nc; ++c)
+    for(int i = 0; i < n3; ++i)
+    {
+      int64_t const g = space.node(c, i);
+      u[c * n3 + i]   = space.constrained[g] ? 0.0 : src[g];
+    }
+  cells->cell_loop(v.data(), u.data());
+  std::fill(dst, dst + space.n_nodes, 0.0);
+  for(int64_t c = 0; c < nc; ++c)
+    for(int i = 0; i < n3; ++i)
+      dst[space.node(c, i)] += v[c * n3 + i];
+  for(int64_t g = 0; g < space.n_nodes; ++g)
+    if(space.constrained[g])
+      dst[g] = src[g];
+}
+
+void CGOperator::diagonal(double * dg) const
+{
+  // local diagonal by probing the cell integral with unit vectors (one per local index,
+  // in all cells at once), summed over the cells sharing a node
+  int64_t const       nc = space.d.n_cells[0] * (int64_t)space.d.n_cells[1] * space.d.n_cells[2];
+  int const           n3 = (space.p + 1) * (space.p + 1) * (space.p + 1);
+  std::vector<double> e((size_t)nc * n3), out((size_t)nc * n3);
+  std::fill(dg, dg + space.n_nodes, 0.0);
+  for(int i = 0; i < n3; ++i)
+  {
+    std::fill(e.begin(), e.end(), 0.0);
+    std::fill(out.begin(), out.end(), 0.0);
+    for(int64_t c = 0; c < nc; ++c)
+      e[c * n3 + i] = 1.0;
+    cells->cell_loop(out.data(), e.data());
+    for(int64_t c = 0; c < nc; ++c)
+      dg[space.node(c, i)] += out[c * n3 + i];
+  }
+  for(int64_t g = 0; g < space.n_nodes; ++g)
+    if(space.constrained[g])
+      dg[g] = 1.0;
+}
+
 // ---------------------------------------------------------------- Chebyshev
 
 static Apply jacobi(std::vector<double> const & inv_diag)
@@ -370,34 +471,164 @@ void Chebyshev::step(std::vector<double> & x, std::vector<double> const & b) con
 
 // ---------------------------------------------------------------- multigrid
 
+std::vector<LevelInfo> mg_level_list(int type, std::vector<int> const & degrees, int n_h, bool coarse_q1)
+{
+  if(type == mg_legacy)
+    type = coarse_q1 ? mg_pcMG : mg_pMG;
+  auto has = [&](std::initializer_list<int> l) {
+    for(int t : l)
+      if(t == type)
+        return true;
+    return false;
+  };
+  bool const h_involved = has({mg_hMG, mg_chMG, mg_hcMG, mg_hpMG, mg_chpMG, mg_hcpMG, mg_hpcMG,
+                               mg_phMG, mg_cphMG, mg_pchMG, mg_phcMG});
+  int const  H          = h_involved ? std::max(1, n_h) : 1;
+  int const  k          = degrees.at(0);
+  // p-levels (degree, dg), built fine to coarse and then reversed
+  std::vector<LevelInfo> pl;
+  if(type == mg_hMG)
+    pl.push_back({0, k, true});
+  else if(has({mg_cMG, mg_chMG, mg_hcMG}))
+  {
+    pl.push_back({0, k, false});
+    pl.push_back({0, k, true});
+  }
+  else
+  {
+    bool dg = true;
+    if(has({mg_cpMG, mg_hcpMG, mg_chpMG, mg_cphMG}))  // c-transfer before p-coarsening
+    {
+      pl.push_back({0, k, true});
+      dg = false;
+    }
+    for(int p : degrees)
+      pl.push_back({0, p, dg});
+    if(has({mg_pcMG, mg_hpcMG, mg_phcMG, mg_pchMG}))  // c-transfer after p-coarsening
+      pl.push_back({0, degrees.back(), false});
+    std::reverse(pl.begin(), pl.end());
+  }
+  std::vector<LevelInfo> L;
+  auto at = [](int h, LevelInfo p) { return LevelInfo{h, p.degree, p.dg}; };
+  int const P = (int)pl.size();
+  if(type == mg_hMG)
+    for(int h = 0; h < H; ++h)
+      L.push_back(at(h, pl.front()));
+  else if(type == mg_cMG)
+  {
+    L.push_back(at(H - 1, pl.front()));
+    L.push_back(at(H - 1, pl.back()));
+  }
+  else if(type == mg_chMG)
+  {
+    for(int h = 0; h < H; ++h)
+      L.push_back(at(h, pl.front()));
+    L.push_back(at(H - 1, pl.back()));
+  }
+  else if(type == mg_hcMG)
+  {
+    L.push_back(at(0, pl.front()));
+    for(int h = 0; h < H; ++h)
+      L.push_back(at(h, pl.back()));
+  }
+  else if(has({mg_pMG, mg_pcMG, mg_cpMG}))
+    for(auto const & p : pl)
+      L.push_back(at(0, p));
+  else if(has({mg_phMG, mg_cphMG, mg_pchMG}))
+  {
+    for(int h = 0; h + 1 < H; ++h)
+      L.push_back(at(h, pl.front()));
+    for(auto const & p : pl)
+      L.push_back(at(H - 1, p));
+  }
+  else if(has({mg_hpMG, mg_hcpMG, mg_hpcMG}))
+  {
+    for(int i = 0; i + 1 < P; ++i)
+      L.push_back(at(0, pl[i]));
+    for(int h = 0; h < H; ++h)
+      L.push_back(at(h, pl.back()));
+  }
+  else if(type == mg_phcMG)
+  {
+    L.push_back(at(0, pl.front()));
+    for(int h = 0; h + 1 < H; ++h)
+      L.push_back(at(h, pl[1]));
+    for(int i = 1; i < P; ++i)
+      L.push_back(at(H - 1, pl[i]));
+  }
+  else if(type == mg_chpMG)
+  {
+    for(int i = 0; i + 2 < P; ++i)
+      L.push_back(at(0, pl[i]));
+    for(int h = 0; h < H; ++h)
+      L.push_back(at(h, pl[P - 2]));
+    L.push_back(at(H - 1, pl.back()));
+  }
+  else
+    throw std::runtime_error("unknown multigrid type");
+  for(size_t l = 1; l < L.size(); ++l)
+  {
+    int const changes = (L[l].h != L[l - 1].h) + (L[l].degree != L[l - 1].degree) +
+                        (L[l].dg != L[l - 1].dg);
+    if(changes != 1)
+      throw std::runtime_error("between two multigrid levels exactly one of h, p, c must change");
+  }
+  return L;
+}
+
+static MeshDesc coarsened(MeshDesc m, int times)
+{
+  for(int k = 0; k < 3; ++k)
+  {
+    if(m.n_cells[k] % (1 << times) != 0)
+      throw std::runtime_error("h-levels need cell counts divisible by 2 per coarsening");
+    m.n_cells[k] >>= times;
+  }
+  return m;
+}
+
 Multigrid::Multigrid(MeshDesc const & m, MGDesc const & d, double ip) : mesh(m), desc(d)
 {
-  int const nl = (int)d.degrees.size() + (d.coarse_q1 ? 1 : 0);
+  std::vector<LevelInfo> const info = mg_level_list(d.type, d.degrees, d.n_h_levels, d.coarse_q1);
+  int const                    nl   = (int)info.size();
+  int const                    H    = info.back().h + 1;
   levels.resize(nl);
-  int l = 0;
-  if(d.coarse_q1)
-  {
-    levels[0].q1_op  = std::make_unique<Q1Operator>(m);
-    levels[0].dg     = false;
-    levels[0].degree = 1;
-    levels[0].n      = levels[0].q1_op->n_nodes;
-    Q1Operator * q   = levels[0].q1_op.get();
-    levels[0].A      = [q](double * dst, double const * src) { q->vmult(dst, src); };
-    levels[0].inv_diag.resize(levels[0].n);
-    q->diagonal(levels[0].inv_diag.data());
-    invert_diagonal(levels[0].inv_diag);
-    ++l;
-  }
-  for(int i = (int)d.degrees.size() - 1; i >= 0; --i, ++l)
+  for(int l = 0; l < nl; ++l)
   {
     Level & L = levels[l];
-    L.degree  = d.degrees[i];
-    L.dg_op   = std::make_unique<LaplaceOperator>(m, L.degree, ip);
-    L.n       = L.dg_op->n_dofs();
-    LaplaceOperator * op = L.dg_op.get();
-    L.A                  = [op](double * dst, double const * src) { op->vmult(dst, src); };
-    L.inv_diag.resize(L.n);
-    op->diagonal(L.inv_diag.data());
+    L.h       = info[l].h;
+    L.degree  = info[l].degree;
+    L.dg      = info[l].dg;
+    L.mesh    = coarsened(m, H - 1 - L.h);
+    if(L.dg)
+    {
+      L.dg_op              = std::make_unique<LaplaceOperator>(L.mesh, L.degree, ip);
+      L.n                  = L.dg_op->n_dofs();
+      LaplaceOperator * op = L.dg_op.get();
+      L.A                  = [op](double * dst, double const * src) { op->vmult(dst, src); };
+      L.inv_diag.resize(L.n);
+      op->diagonal(L.inv_diag.data());
+    }
+    else
+    {
+      L.space = CGSpace(L.mesh, L.degree);
+      L.n     = L.space.n_nodes;
+      L.inv_diag.resize(L.n);
+      if(L.degree == 1)
+      {
+        L.q1_op        = std::make_unique<Q1Operator>(L.mesh);
+        Q1Operator * q = L.q1_op.get();
+        L.A            = [q](double * dst, double const * src) { q->vmult(dst, src); };
+        q->diagonal(L.inv_diag.data());
+      }
+      else
+      {
+        L.cg_op        = std::make_unique<CGOperator>(L.mesh, L.degree, ip);
+        CGOperator * q = L.cg_op.get();
+        L.A            = [q](double * dst, double const * src) { q->vmult(dst, src); };
+        q->diagonal(L.inv_diag.data());
+      }
+    }
     invert_diagonal(L.inv_diag);
   }
   for(int k = 0; k < nl; ++k)
@@ -406,21 +637,38 @@ Multigrid::Multigrid(MeshDesc const & m, MGDesc const & d, double ip) : mesh(m),
     L.defect.assign(L.n, 0.0);
     L.sol.assign(L.n, 0.0);
     L.t.assign(L.n, 0.0);
-    if(k > 0)
+    if(k == 0)
+      continue;
+    std::vector<char> mask;
+    L.smoother.setup_smoother(L.A, L.inv_diag, mask, d.smoother_degree, d.smoothing_range,
+                              d.eig_its);
+    Level const & C = levels[k - 1];
+    if(C.dg != L.dg)
+      L.transfer = 'c';
+    else if(C.degree != L.degree)
     {
-      std::vector<char> mask;
-      L.smoother.setup_smoother(L.A, L.inv_diag, mask, d.smoother_degree, d.smoothing_range,
-                                d.eig_its);
-      if(levels[k - 1].dg)
+      L.transfer = 'p';
+      int const           nf = L.degree + 1, nc = C.degree + 1;
+      std::vector<double> xf, xc;
+      gauss_lobatto(nf, xf);
+      gauss_lobatto(nc, xc);
+      L.P.resize(nf * nc);
+      for(int a = 0; a < nf; ++a)
+        for(int i = 0; i < nc; ++i)
+          L.P[a * nc + i] = lagrange(xc, i, xf[a]);
+    }
+    else
+    {
+      L.transfer = 'h';
+      int const           n = L.degree + 1;
+      std::vector<double> x;
+      gauss_lobatto(n, x);
+      for(int b = 0; b < 2; ++b)
       {
-        int const           nf = L.degree + 1, nc = levels[k - 1].degree + 1;
-        std::vector<double> xf, xc;
-        gauss_lobatto(nf, xf);
-        gauss_lobatto(nc, xc);
-        L.P.resize(nf * nc);
-        for(int a = 0; a < nf; ++a)
-          for(int i = 0; i < nc; ++i)
-            L.P[a * nc + i] = lagrange(xc, i, xf[a]);
+        L.Ph[b].resize(n * n);
+        for(int a = 0; a < n; ++a)
+          for(int i = 0; i < n; ++i)
+            L.Ph[b][a * n + i] = lagrange(x, i, 0.5 * (b + x[a]));
       }
     }
   }
@@ -428,70 +676,153 @@ Multigrid::Multigrid(MeshDesc const & m, MGDesc const & d, double ip) : mesh(m),
     coarse_cheb.setup_coarse(levels[0].A, levels[0].inv_diag, d.coarse_rel_tol, d.singular);
 }
 
-void Multigrid::prolongate_add(int l, double * fine, double const * coarse) const
+// Cell-wise transfer between level l-1 (coarse) and l (fine). For every fine cell, the coarse
+// values of its parent cell (the cell itself for 'p') are interpolated with the 1D matrices of
+// the cell's position in the parent. On continuous levels the values are gathered from and
+// scattered to global nodes: prolongation writes each fine node once (every cell sharing it
+// computes the same value), restriction weights each fine node by 1/valence so that the sum over
+// the cells sharing it counts it once -- restriction is then the transpose of prolongation.
+// Constrained nodes read and receive zero.
+namespace
 {
-  Level const & F = levels[l];
-  Level const & C = levels[l - 1];
-  if(C.dg)
+struct CellTransfer
+{
+  int                 nf, nc;  // 1D sizes
+  double const *      M[3];    // per direction nf x nc
+};
+
+void parent_and_matrices(Multigrid::Level const & F, Multigrid::Level const & C, int64_t c,
+                         int64_t & parent, CellTransfer & T)
+{
+  int const nx = F.mesh.n_cells[0], ny = F.mesh.n_cells[1];
+  int const cc[3] = {(int)(c % nx), (int)((c / nx) % ny), (int)(c / ((int64_t)nx * ny))};
+  T.nf            = F.degree + 1;
+  T.nc            = C.degree + 1;
+  if(F.transfer == 'h')
   {
-    int const nf = F.degree + 1, nc = C.degree + 1;
-    int const nf3 = nf * nf * nf, nc3 = nc * nc * nc;
-#pragma omp parallel for schedule(static)
-    for(int64_t c = 0; c < mesh.n_cells[0] * (int64_t)mesh.n_cells[1] * mesh.n_cells[2]; ++c)
-    {
-      std::vector<double> out(nf3);
-      tensor_apply(nf, nf, nf, nc, F.P.data(), F.P.data(), F.P.data(), coarse + c * nc3, out.data());
-      for(int i = 0; i < nf3; ++i)
-        fine[c * nf3 + i] += out[i];
-    }
+    int const pc[3] = {cc[0] / 2, cc[1] / 2, cc[2] / 2};
+    parent = pc[0] + (int64_t)C.mesh.n_cells[0] * (pc[1] + (int64_t)C.mesh.n_cells[1] * pc[2]);
+    for(int k = 0; k < 3; ++k)
+      T.M[k] = F.Ph[cc[k] & 1].data();
   }
   else
   {
-    Q1Operator const & q  = *C.q1_op;
-    int64_t const      nc = q.mesh.n_cells;
+    parent = c;
+    for(int k = 0; k < 3; ++k)
+      T.M[k] = F.P.data();
+  }
+}
+}  // namespace
+
+void Multigrid::prolongate_add(int l, double * fine, double const * coarse) const
+{
+  Level const & F  = levels[l];
+  Level const & C  = levels[l - 1];
+  int64_t const nc = F.mesh.n_cells[0] * (int64_t)F.mesh.n_cells[1] * F.mesh.n_cells[2];
+  if(F.transfer == 'c')  // DG_p += continuous Q_p node values
+  {
+    int const n3 = (F.degree + 1) * (F.degree + 1) * (F.degree + 1);
+    for(int64_t c = 0; c < nc; ++c)
+      for(int i = 0; i < n3; ++i)
+      {
+        int64_t const g = C.space.node(c, i);
+        if(!C.space.constrained[g])
+          fine[c * n3 + i] += coarse[g];
+      }
+    return;
+  }
+  int const nf3 = (F.degree + 1) * (F.degree + 1) * (F.degree + 1);
+  int const nc3 = (C.degree + 1) * (C.degree + 1) * (C.degree + 1);
+  if(F.dg)
+  {
+#pragma omp parallel for schedule(static)
     for(int64_t c = 0; c < nc; ++c)
     {
-      int cc[3];
-      q.mesh.cell_coords(c, cc[0], cc[1], cc[2]);
-      for(int v = 0; v < 8; ++v)
-      {
-        int64_t const node = q.node_of(cc, v);
-        if(!q.constrained[node])
-          fine[c * 8 + v] += coarse[node];
-      }
+      int64_t      parent;
+      CellTransfer T;
+      parent_and_matrices(F, C, c, parent, T);
+      std::vector<double> out(nf3);
+      tensor_apply(T.nf, T.nf, T.nf, T.nc, T.M[0], T.M[1], T.M[2], coarse + parent * nc3, out.data());
+      for(int i = 0; i < nf3; ++i)
+        fine[c * nf3 + i] += out[i];
     }
+    return;
   }
+  std::vector<double> value(F.n, 0.0);
+  std::vector<double> in(nc3), out(nf3);
+  for(int64_t c = 0; c < nc; ++c)
+  {
+    int64_t      parent;
+    CellTransfer T;
+    parent_and_matrices(F, C, c, parent, T);
+    for(int i = 0; i < nc3; ++i)
+    {
+      int64_t const g = C.space.node(parent, i);
+      in[i]           = C.space.constrained[g] ? 0.0 : coarse[g];
+    }
+    tensor_apply(T.nf, T.nf, T.nf, T.nc, T.M[0], T.M[1], T.M[2], in.data(), out.data());
+    for(int i = 0; i < nf3; ++i)
+      value[F.space.node(c, i)] = out[i];
+  }
+  for(int64_t g = 0; g < F.n; ++g)
+    if(!F.space.constrained[g])
+      fine[g] += value[g];
 }
 
 void Multigrid::restrict_add(int l, double * coarse, double const * fine) const
 {
-  Level const & F = levels[l];
-  Level const & C = levels[l - 1];
-  if(C.dg)
+  Level const & F  = levels[l];
+  Level const & C  = levels[l - 1];
+  int64_t const nc = F.mesh.n_cells[0] * (int64_t)F.mesh.n_cells[1] * F.mesh.n_cells[2];
+  if(F.transfer == 'c')  // continuous node += sum over the cells sharing it
   {
-    int const nf = F.degree + 1, nc = C.degree + 1;
-    int const nf3 = nf * nf * nf, nc3 = nc * nc * nc;
-#pragma omp parallel for schedule(static)
-    for(int64_t c = 0; c < mesh.n_cells[0] * (int64_t)mesh.n_cells[1] * mesh.n_cells[2]; ++c)
-      tensor_apply_T(nf, nf, nf, nc, F.P.data(), F.P.data(), F.P.data(), fine + c * nf3,
-                     coarse + c * nc3);
+    int const n3 = (F.degree + 1) * (F.degree + 1) * (F.degree + 1);
+    for(int64_t c = 0; c < nc; ++c)
+      for(int i = 0; i < n3; ++i)
+      {
+        int64_t const g = C.space.node(c, i);
+        if(!C.space.constrained[g])
+          coarse[g] += fine[c * n3 + i];
+      }
+    return;
   }
-  else
+  int const nf3 = (F.degree + 1) * (F.degree + 1) * (F.degree + 1);
+  int const nc3 = (C.degree + 1) * (C.degree + 1) * (C.degree + 1);
+  if(F.dg)
   {
-    Q1Operator const & q  = *C.q1_op;
-    int64_t const      nc = q.mesh.n_cells;
+    // serial: several fine cells of an h-transfer add into one parent
+    std::vector<double> out(nc3);
     for(int64_t c = 0; c < nc; ++c)
     {
-      int cc[3];
-      q.mesh.cell_coords(c, cc[0], cc[1], cc[2]);
-      for(int v = 0; v < 8; ++v)
-      {
-        int64_t const node = q.node_of(cc, v);
-        if(!q.constrained[node])
-          coarse[node] += fine[c * 8 + v];
-      }
+      int64_t      parent;
+      CellTransfer T;
+      parent_and_matrices(F, C, c, parent, T);
+      std::fill(out.begin(), out.end(), 0.0);
+      tensor_apply_T(T.nf, T.nf, T.nf, T.nc, T.M[0], T.M[1], T.M[2], fine + c * nf3, out.data());
+      for(int i = 0; i < nc3; ++i)
+        coarse[parent * nc3 + i] += out[i];
     }
+    return;
   }
+  std::vector<double> in(nf3), out(nc3), acc(C.n, 0.0);
+  for(int64_t c = 0; c < nc; ++c)
+  {
+    int64_t      parent;
+    CellTransfer T;
+    parent_and_matrices(F, C, c, parent, T);
+    for(int i = 0; i < nf3; ++i)
+    {
+      int64_t const g = F.space.node(c, i);
+      in[i]           = F.space.constrained[g] ? 0.0 : fine[g] / F.space.valence(g);
+    }
+    std::fill(out.begin(), out.end(), 0.0);
+    tensor_apply_T(T.nf, T.nf, T.nf, T.nc, T.M[0], T.M[1], T.M[2], in.data(), out.data());
+    for(int i = 0; i < nc3; ++i)
+      acc[C.space.node(parent, i)] += out[i];
+  }
+  for(int64_t g = 0; g < C.n; ++g)
+    if(!C.space.constrained[g])
+      coarse[g] += acc[g];
 }
 
 void Multigrid::v_cycle(int l) const

@@ -1,5 +1,6 @@
-// CPU oracle multigrid (TEST INFRASTRUCTURE ONLY): hybrid p-multigrid DG_k -> ... -> DG_1 ->
-// continuous Q1 with Chebyshev-Jacobi smoothing, FP64.
+// CPU oracle multigrid (TEST INFRASTRUCTURE ONLY): hybrid multigrid over h-levels (coarser
+// meshes), p-levels (lower degrees) and c-levels (continuous FE_Q) with Chebyshev-Jacobi
+// smoothing, FP64.
 //
 // Reference files followed:
 //   V-cycle            : solvers_and_preconditioners/multigrid/multigrid_algorithm.h:87-108,172-240
@@ -9,7 +10,9 @@
 //                        PreconditionChebyshev: eigenvalue estimate by CG/Lanczos, recurrence)
 //   coarse solvers     : multigrid/coarse_grid_solvers.h:61-321,
 //                        utilities/compute_eigenvalues.h:38-88
-//   continuous Q1 level: multigrid/constraints.h:109-194, operators/operator_base.cpp:299-309
+//   continuous levels : multigrid/constraints.h:109-194, operators/operator_base.cpp:299-309
+//   level list (types) : multigrid/multigrid_preconditioner_base.cpp:97-323,
+//                        multigrid/multigrid_parameters.h:39-65
 #pragma once
 #include <memory>
 #include <vector>
@@ -33,6 +36,36 @@ struct Q1Operator
   void    diagonal(double * d) const;
 };
 
+// Continuous FE_Q(p) node numbering of a structured box (GL support points): (n_d p + 1) nodes
+// per non-periodic direction, n_d p per periodic one; Dirichlet sides constrained.
+struct CGSpace
+{
+  MeshDesc          d;
+  int               p = 1;
+  int               nn[3];
+  int64_t           n_nodes = 0;
+  std::vector<char> constrained;
+  CGSpace() = default;
+  CGSpace(MeshDesc const & d, int p);
+  // global node of local node (i, j, l) (lexicographic index `local`) of cell c
+  int64_t node(int64_t c, int local) const;
+  // number of cells sharing node g
+  int valence(int64_t g) const;
+};
+
+// continuous Q_p Laplace on the structured box: the cell integrals of the SIPG operator (its
+// face terms vanish on continuous functions), assembled over cells through the node map;
+// Dirichlet nodes read as zero and act as identity rows
+struct CGOperator
+{
+  CGSpace                          space;
+  std::unique_ptr<LaplaceOperator> cells;  // same mesh and degree; only cell_loop is used
+  CGOperator(MeshDesc const & d, int p, double ip_factor);
+  int64_t n() const { return space.n_nodes; }
+  void    vmult(double * dst, double const * src) const;
+  void    diagonal(double * d) const;
+};
+
 struct Chebyshev
 {
   Apply               A;
@@ -49,10 +82,32 @@ struct Chebyshev
   void step(std::vector<double> & x, std::vector<double> const & b) const;
 };
 
+// MultigridType (multigrid_parameters.h:39-57), numbered as there; 0 = the legacy choice
+// p then c (pcMG) when coarse_q1, else pMG
+enum MGType
+{
+  mg_legacy = 0, mg_hMG, mg_chMG, mg_hcMG, mg_cMG, mg_pMG, mg_cpMG, mg_pcMG, mg_hpMG, mg_chpMG,
+  mg_hcpMG, mg_hpcMG, mg_phMG, mg_cphMG, mg_pchMG, mg_phcMG
+};
+
+struct LevelInfo
+{
+  int  h      = 0;  // mesh level: 0 coarsest
+  int  degree = 1;
+  bool dg     = true;
+};
+
+// MultigridPreconditionerBase::initialize_levels: levels coarse to fine from the type, the
+// p-sequence (degrees fine to coarse, degrees[0] = operator degree) and the number of meshes
+std::vector<LevelInfo> mg_level_list(int type, std::vector<int> const & degrees, int n_h_levels,
+                                     bool coarse_q1);
+
 struct MGDesc
 {
   std::vector<int> degrees;       // fine to coarse
   bool             coarse_q1 = true;
+  int              type = mg_legacy;
+  int              n_h_levels = 1;  // meshes of the h-levels (finest included)
   int              smoother_degree = 5;
   double           smoothing_range = 20;
   int              eig_its = 20;
@@ -66,15 +121,23 @@ struct Multigrid
 {
   struct Level
   {
+    int                                     h      = 0;
     int                                     degree = 1;
     bool                                    dg     = true;
     int64_t                                 n      = 0;
+    MeshDesc                                mesh;
+    CGSpace                                 space;  // continuous levels
     std::unique_ptr<LaplaceOperator>        dg_op;
     std::unique_ptr<Q1Operator>             q1_op;
+    std::unique_ptr<CGOperator>             cg_op;
     Apply                                   A;
     std::vector<double>                     inv_diag;
     Chebyshev                               smoother;
-    std::vector<double>                     P;  // nf x nc to the next coarser DG level
+    // transfer to the next coarser level: 'p' (degree), 'h' (mesh) or 'c' (DG -> continuous)
+    char                                    transfer = 0;
+    // 1D interpolation matrices (n_fine x n_coarse, row-major): P for 'p'; Ph[b] for the
+    // child with bit b of 'h' (fine coordinate (b + xi)/2 in the parent)
+    std::vector<double>                     P, Ph[2];
     mutable std::vector<double>             defect, sol, t;
   };
   MeshDesc           mesh;

@@ -219,6 +219,59 @@ def assemble(mesh, degree, ip=1.0, omega=None, plant=None):
     return A, rhs
 
 
+def continuous_nodes(mesh, degree, plant=None):
+    """Global node of every (cell, local GL node) of the continuous FE_Q(degree) space on the
+    box, x fastest: (n_d degree + 1) nodes per non-periodic direction, n_d degree per periodic
+    one; and the Dirichlet mask. Returns (node_of [n_cells, n^3], n_nodes, constrained)."""
+    n = degree + 1
+    nn = [int(mesh.nc[d]) * degree + (0 if mesh.periodic[d] else 1) for d in range(3)]
+    node_of = np.zeros((len(mesh.cells), n ** 3), dtype=np.int64)
+    for cc in mesh.cells:
+        ci = mesh.index(cc)
+        for l, (k, j, i) in enumerate(itertools.product(range(n), range(n), range(n))):
+            if plant == "swap_xy":  # a plausible mistake: local node order y fastest
+                i, j = j, i
+            g = [cc[0] * degree + i, cc[1] * degree + j, cc[2] * degree + k]
+            g = [g[d] % nn[d] if mesh.periodic[d] else g[d] for d in range(3)]
+            node_of[ci, l] = g[0] + nn[0] * (g[1] + nn[1] * g[2])
+    N = nn[0] * nn[1] * nn[2]
+    constrained = np.zeros(N, dtype=bool)
+    for g in range(N):
+        c = (g % nn[0], (g // nn[0]) % nn[1], g // (nn[0] * nn[1]))
+        for d in range(3):
+            if mesh.periodic[d]:
+                continue
+            if (c[d] == 0 and mesh.bc[2 * d] == 0) or (c[d] == nn[d] - 1 and mesh.bc[2 * d + 1] == 0):
+                constrained[g] = True
+    return node_of, N, constrained
+
+
+def assemble_continuous(mesh, degree, plant=None):
+    """dense continuous Q_degree Laplace: cell stiffness matrices (full QGauss(degree+1)
+    quadrature) summed through the node map; constrained rows and columns replaced by the
+    identity (Dirichlet nodes). plant="swap_xy" assembles with a wrong local node order."""
+    n = degree + 1
+    b1 = Basis1D(gl_nodes(n))
+    gp, gw = gauss(n)
+    qx = np.array([[gp[i], gp[j], gp[k]] for k in range(n) for j in range(n) for i in range(n)])
+    qw = np.array([gw[i] * gw[j] * gw[k] for k in range(n) for j in range(n) for i in range(n)])
+    _, gref = _cell_basis(b1, qx)
+    node_of, N, constrained = continuous_nodes(mesh, degree, plant)
+    A = np.zeros((N, N))
+    for cc in mesh.cells:
+        _, J = mesh.map(cc, qx)
+        Ji = np.linalg.inv(J)
+        jxw = np.linalg.det(J) * qw
+        gphys = np.einsum("qji,qmj->qmi", Ji, gref)
+        K = np.einsum("q,qmi,qli->ml", jxw, gphys, gphys)
+        idx = node_of[mesh.index(cc)]
+        np.add.at(A, (idx[:, None], idx[None, :]), K)  # duplicates on a one-cell periodic seam
+    A[constrained, :] = 0.0
+    A[:, constrained] = 0.0
+    A[constrained, constrained] = 1.0
+    return A
+
+
 def assemble_divergence(mesh, degree_p, degree_u):
     """Dense weak DG divergence B (pressure x velocity) of the pressure step
     (divergence_operator.cpp:168-298, weak form, central flux):
