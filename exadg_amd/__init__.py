@@ -57,7 +57,14 @@ class MGDesc(ctypes.Structure):
         ("coarse_max_iter", ctypes.c_int32),
         ("operator_is_singular", ctypes.c_int32),
         ("capture_graph", ctypes.c_int32),
+        ("mg_type", ctypes.c_int32),
+        ("n_h_levels", ctypes.c_int32),
     ]
+
+
+# MultigridType (exadg_mg_type), numbered as in the reference; "legacy" = pcMG/pMG by coarse_q1
+MG_TYPES = ["legacy", "hMG", "chMG", "hcMG", "cMG", "pMG", "cpMG", "pcMG", "hpMG", "chpMG",
+            "hcpMG", "hpcMG", "phMG", "cphMG", "pchMG", "phcMG"]
 
 
 class SolverData(ctypes.Structure):
@@ -78,8 +85,10 @@ COARSE_CG, COARSE_CHEBYSHEV = 0, 1
 
 def mg_desc(degree, degrees=None, precision=FP32, coarse_q1=True, coarse_solver=COARSE_CG,
             coarse_rel_tol=1e-3, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=20,
-            singular=False, capture_graph=False):
-    """MultigridData; default p-sequence Bisect (k, k/2, ..., 1) + continuous Q1 level."""
+            singular=False, capture_graph=False, mg_type="legacy", n_h_levels=0):
+    """MultigridData; default p-sequence Bisect (k, k/2, ..., 1) + continuous Q1 level.
+    mg_type: a name of MG_TYPES (or its index); n_h_levels: meshes of the h-levels (0 =
+    automatic)."""
     d = MGDesc()
     check(lib().exadg_mg_desc_default(degree, ctypes.byref(d)))
     if degrees is not None:
@@ -95,6 +104,8 @@ def mg_desc(degree, degrees=None, precision=FP32, coarse_q1=True, coarse_solver=
     d.eig_cg_iterations = eig_cg_iterations
     d.operator_is_singular = int(singular)
     d.capture_graph = int(capture_graph)
+    d.mg_type = MG_TYPES.index(mg_type) if isinstance(mg_type, str) else int(mg_type)
+    d.n_h_levels = int(n_h_levels)
     return d
 
 
@@ -153,6 +164,7 @@ def lib():
             "exadg_preconditioner_smooth": [vp, vp, vp, vp, vp],
             "exadg_preconditioner_destroy": [vp],
             "exadg_preconditioner_mg_info": [vp, ip, ctypes.POINTER(ctypes.c_double), ip, ip],
+            "exadg_preconditioner_mg_levels": [vp, ctypes.c_int, ip, ip, ip, ip, i64p],
             "exadg_cg_solve": [vp, vp, vp, vp, ctypes.POINTER(SolverData), ctypes.POINTER(SolveInfo),
                                vp],
             "exadg_cg_solve_noalloc": [vp, vp, vp, vp, ctypes.POINTER(SolverData),
@@ -391,6 +403,14 @@ class Preconditioner:
         check_vector(b, self.n, torch.float64, "b")
         check_vector(x, self.n, torch.float64, "x")
         check(lib().exadg_preconditioner_smooth(op.handle, self._h, _ptr(b), _ptr(x), _stream(stream)))
+
+    def mg_levels(self):
+        """[(h, degree, dg, n)] per level, coarse to fine (h = 0: the coarsest mesh)."""
+        nl = ctypes.c_int()
+        h, deg, dg = (ctypes.c_int * 64)(), (ctypes.c_int * 64)(), (ctypes.c_int * 64)()
+        n = (ctypes.c_int64 * 64)()
+        check(lib().exadg_preconditioner_mg_levels(self._h, 64, ctypes.byref(nl), h, deg, dg, n))
+        return [(h[i], deg[i], bool(dg[i]), n[i]) for i in range(nl.value)]
 
     def mg_info(self):
         nl = ctypes.c_int()

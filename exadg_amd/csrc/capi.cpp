@@ -208,6 +208,26 @@ void mg_info(ExaDG::PreconditionerBase const * p, int * n_levels, double * lmax,
   if(coarse_its)
     *coarse_its = mg.last_coarse_iterations();
 }
+
+template<typename Number>
+void mg_levels(ExaDG::PreconditionerBase const * p, int max_levels, int * n_levels, int * h,
+               int * degree, int * dg, int64_t * n)
+{
+  auto const & mg = static_cast<ExaDG::MultigridPreconditioner<Number> const &>(*p);
+  *n_levels       = mg.n_levels();
+  for(int l = 0; l < mg.n_levels() && l < max_levels; ++l)
+  {
+    ExaDG::MGLevelInfo const i = mg.level_info(l);
+    if(h)
+      h[l] = i.h;
+    if(degree)
+      degree[l] = i.degree;
+    if(dg)
+      dg[l] = i.dg ? 1 : 0;
+    if(n)
+      n[l] = mg.level_operator(l).n();
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -375,6 +395,10 @@ int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_des
       ExaDG::MultigridData d;
       d.degrees.assign(mg->dg_degrees, mg->dg_degrees + mg->n_dg_levels);
       d.coarse_q1            = mg->coarse_q1 != 0;
+      require(mg->mg_type >= EXADG_MG_LEGACY && mg->mg_type <= EXADG_MG_PHCMG, "unknown mg_type");
+      require(mg->n_h_levels >= 0 && mg->n_h_levels <= 16, "n_h_levels out of range");
+      d.type                 = static_cast<ExaDG::MultigridType>(mg->mg_type);
+      d.n_h_levels           = mg->n_h_levels;
       d.smoother_degree      = mg->smoother_degree;
       d.smoothing_range      = mg->smoothing_range;
       d.eig_cg_iterations    = mg->eig_cg_iterations;
@@ -449,6 +473,19 @@ int exadg_preconditioner_mg_info(const exadg_preconditioner * p, int * n_levels,
       mg_info<float>(p->p.get(), n_levels, lmax, cheb_degree, coarse_its);
     else
       mg_info<double>(p->p.get(), n_levels, lmax, cheb_degree, coarse_its);
+  });
+}
+
+int exadg_preconditioner_mg_levels(const exadg_preconditioner * p, int max_levels, int * n_levels,
+                                   int * h, int * degree, int * dg, int64_t * n)
+{
+  return guarded([&] {
+    require(p && n_levels, "null argument");
+    require((p->kind & 0xff) == EXADG_PRECOND_MULTIGRID, "not a multigrid preconditioner");
+    if(p->kind & 0x100)
+      mg_levels<float>(p->p.get(), max_levels, n_levels, h, degree, dg, n);
+    else
+      mg_levels<double>(p->p.get(), max_levels, n_levels, h, degree, dg, n);
   });
 }
 

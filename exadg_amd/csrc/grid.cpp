@@ -62,7 +62,7 @@ double inverse3(double const * J, double * Ji)
 }  // namespace
 
 StructuredGrid::StructuredGrid(exadg_mesh_desc const & mesh, exadg_bc_desc const & bc)
-  : mesh_(mesh)
+  : mesh_(mesh), bc_(bc)
 {
   for(int d = 0; d < 3; ++d)
   {
@@ -96,6 +96,19 @@ StructuredGrid::StructuredGrid(exadg_mesh_desc const & mesh, exadg_bc_desc const
       EXADG_THROW(EXADG_ERR_ARGUMENT, "mapping_degree must be in [1,15] for deformed meshes");
     gauss_lobatto_points(mesh.mapping_degree + 1, map_nodes_);
   }
+}
+
+StructuredGrid StructuredGrid::coarsened(int times) const
+{
+  exadg_mesh_desc m = mesh_;
+  for(int d = 0; d < 3; ++d)
+  {
+    if(times < 0 || m.n_cells[d] % (1 << times) != 0)
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "h-levels need cells-per-direction counts divisible by 2 "
+                                      "for every coarsening");
+    m.n_cells[d] >>= times;
+  }
+  return StructuredGrid(m, bc_);
 }
 
 void StructuredGrid::push_forward(double const * p, double * x) const

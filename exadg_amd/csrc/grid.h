@@ -21,6 +21,10 @@ public:
   StructuredGrid(exadg_mesh_desc const & mesh, exadg_bc_desc const & bc);
 
   exadg_mesh_desc const & mesh() const { return mesh_; }
+  exadg_bc_desc const &   bc() const { return bc_; }
+  // the same box with every cells-per-direction count halved `times` times (h-multigrid
+  // levels); same bounds, deformation, mapping and boundary types
+  StructuredGrid coarsened(int times) const;
   BoxTopology const &     topology() const { return topo_; }
   int64_t                 n_cells() const { return n_cells_; }
   bool                    affine() const { return affine_; }
@@ -35,6 +39,7 @@ public:
 
 private:
   exadg_mesh_desc     mesh_;
+  exadg_bc_desc       bc_;
   BoxTopology         topo_;
   int64_t             n_cells_;
   bool                affine_;

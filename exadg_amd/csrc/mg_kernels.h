@@ -1,5 +1,5 @@
-// Multigrid kernels: polynomial (p) transfer between DG levels on the same mesh, the DG-Q1 <->
-// CG-Q1 (c) transfer, and the continuous Q1 coarse operator.
+// Multigrid kernels: polynomial (p) transfer between DG levels on the same mesh and the
+// continuous Q1 operator (the c-transfer and the other continuous levels are in cg_kernels.h).
 //
 // Reference: MultigridTransfer (solvers_and_preconditioners/multigrid/transfer.cpp:28-96) ->
 // MGTwoLevelTransfer::reinit_polynomial_transfer; the continuous coarse level of the hybrid
@@ -32,14 +32,6 @@ hipError_t launch_p_prolongate_add(Number * dst_f, Number const * src_c, Number 
 template<typename Number>
 hipError_t launch_p_restrict_add(Number * dst_c, Number const * src_f, Number const * P, int nf,
                                  int nc, int64_t n_cells, hipStream_t s);
-// DG1 (8 vertex values per cell, lexicographic) += CG1 node values (constrained nodes read as 0)
-template<typename Number>
-hipError_t launch_c_prolongate_add(Number * dst_dg, Number const * src_cg, BoxTopology const & t,
-                                   Q1Topology const & q, hipStream_t s);
-// CG1 node += sum over the cells sharing it of the DG1 vertex values (constrained nodes skipped)
-template<typename Number>
-hipError_t launch_c_restrict_add(Number * dst_cg, Number const * src_dg, BoxTopology const & t,
-                                 Q1Topology const & q, hipStream_t s);
 // Q1 operator as an assembled 27-point stencil (W: 27 planes of n_nodes weights, plane j for
 // the neighbour offset (j%3-1, (j/3)%3-1, j/9-1), periodic wrap; constrained rows identity)
 template<typename Number>

@@ -75,86 +75,6 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// ---------------------------------------------------------------- Q1 topology helpers
-
-__device__ __forceinline__ bool q1_constrained(Q1Topology const & q, int const * nd)
-{
-#pragma unroll
-  for(int d = 0; d < 3; ++d)
-    if(!q.periodic[d] && ((nd[d] == 0 && q.dirichlet[2 * d]) || (nd[d] == q.nc[d] && q.dirichlet[2 * d + 1])))
-      return true;
-  return false;
-}
-
-__device__ __forceinline__ int64_t q1_node_id(Q1Topology const & q, int const * nd)
-{
-  return nd[0] + (int64_t)q.nn[0] * (nd[1] + (int64_t)q.nn[1] * nd[2]);
-}
-
-// node coordinates of vertex v (bits x,y,z) of the cell at cc, periodic wrap
-__device__ __forceinline__ void q1_vertex_node(Q1Topology const & q, int const * cc, int v, int * nd)
-{
-#pragma unroll
-  for(int d = 0; d < 3; ++d)
-  {
-    int const c = cc[d] + ((v >> d) & 1);
-    nd[d]       = (q.periodic[d] && c == q.nc[d]) ? 0 : c;
-  }
-}
-
-template<typename Number>
-__global__ void __launch_bounds__(256)
-  c_prolongate_kernel(Number * dst_dg, Number const * src_cg, BoxTopology t, Q1Topology q,
-                      int64_t n)
-{
-  int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n)
-    return;
-  int64_t const cell = i / 8;
-  int const     v    = (int)(i % 8);
-  int           cc[3], nd[3];
-  cell_coords(t, cell, cc);
-  q1_vertex_node(q, cc, v, nd);
-  if(!q1_constrained(q, nd))
-    dst_dg[i] += src_cg[q1_node_id(q, nd)];
-}
-
-template<typename Number>
-__global__ void __launch_bounds__(256)
-  c_restrict_kernel(Number * dst_cg, Number const * src_dg, BoxTopology t, Q1Topology q, int64_t n)
-{
-  int64_t const node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(node >= n)
-    return;
-  int nd[3];
-  nd[0]             = (int)(node % q.nn[0]);
-  int64_t const r   = node / q.nn[0];
-  nd[1]             = (int)(r % q.nn[1]);
-  nd[2]             = (int)(r / q.nn[1]);
-  if(q1_constrained(q, nd))
-    return;
-  Number acc = 0;
-  // the (up to) 8 cells sharing the node: cell coordinate nd-1 (vertex bit 1) or nd (bit 0)
-  for(int v = 0; v < 8; ++v)
-  {
-    int  cc[3];
-    bool ok = true;
-#pragma unroll
-    for(int d = 0; d < 3; ++d)
-    {
-      int c = nd[d] - ((v >> d) & 1);
-      if(c < 0)
-        c = q.periodic[d] ? c + q.nc[d] : -1;
-      if(c < 0 || c >= q.nc[d])
-        ok = false;
-      cc[d] = c;
-    }
-    if(ok)
-      acc += src_dg[cell_id(t, cc) * 8 + v];
-  }
-  dst_cg[node] += acc;
-}
-
 // Q1 operator as an assembled 27-point stencil: W[j * n + node] couples node with its
 // neighbour at offset (j % 3 - 1, (j / 3) % 3 - 1, j / 9 - 1) (periodic wrap). Weights to
 // constrained or missing neighbours are zero and those reads are clamped in bounds; a
@@ -257,26 +177,6 @@ hipError_t launch_p_restrict_add(Number * dst_c, Number const * src_f, Number co
 }
 
 template<typename Number>
-hipError_t launch_c_prolongate_add(Number * dst_dg, Number const * src_cg, BoxTopology const & t,
-                                   Q1Topology const & q, hipStream_t s)
-{
-  int64_t const n = (int64_t)t.nc[0] * t.nc[1] * t.nc[2] * 8;
-  hipLaunchKernelGGL((c_prolongate_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, dst_dg,
-                     src_cg, t, q, n);
-  return hipGetLastError();
-}
-
-template<typename Number>
-hipError_t launch_c_restrict_add(Number * dst_cg, Number const * src_dg, BoxTopology const & t,
-                                 Q1Topology const & q, hipStream_t s)
-{
-  int64_t const n = (int64_t)q.nn[0] * q.nn[1] * q.nn[2];
-  hipLaunchKernelGGL((c_restrict_kernel<Number>), dim3(blocks_for(n)), dim3(256), 0, s, dst_cg,
-                     src_dg, t, q, n);
-  return hipGetLastError();
-}
-
-template<typename Number>
 hipError_t launch_q1_stencil(Number * dst, Number const * src, Number const * W,
                              Q1Topology const & q, hipStream_t s)
 {
@@ -303,10 +203,6 @@ hipError_t launch_q1_chebyshev(Number * x_new, Number const * x, Number const * 
                                                  hipStream_t);                                     \
   template hipError_t launch_p_restrict_add<T>(T *, T const *, T const *, int, int, int64_t,       \
                                                hipStream_t);                                       \
-  template hipError_t launch_c_prolongate_add<T>(T *, T const *, BoxTopology const &,              \
-                                                 Q1Topology const &, hipStream_t);                 \
-  template hipError_t launch_c_restrict_add<T>(T *, T const *, BoxTopology const &,                \
-                                               Q1Topology const &, hipStream_t);                   \
   template hipError_t launch_q1_stencil<T>(T *, T const *, T const *, Q1Topology const &,          \
                                            hipStream_t);                                           \
   template hipError_t launch_q1_chebyshev<T>(T *, T const *, T const *, T const *, T const *,      \

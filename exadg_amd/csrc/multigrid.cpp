@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <string>
 
 namespace ExaDG
 {
@@ -202,6 +203,7 @@ Q1Operator<Number>::Q1Operator(StructuredGrid const & grid) : topo_(grid.topolog
       q_.dirichlet[2 * d + s] = (!topo_.periodic[d] && topo_.bc[2 * d + s] == EXADG_BC_DIRICHLET);
   }
   n_nodes_ = (int64_t)q_.nn[0] * q_.nn[1] * q_.nn[2];
+  cg_      = make_cg_topology(topo_, 1);
   bool const shared = grid.affine();
   // element stiffness matrices int grad(phi_a) . grad(phi_b) with QGauss(2)
   std::vector<double> gp, gw;
@@ -357,6 +359,136 @@ std::vector<char> Q1Operator<Number>::constrained_mask() const
   return m;
 }
 
+// ---------------------------------------------------------------- continuous Q_p operator
+
+template<typename Number>
+CGOperator<Number>::CGOperator(StructuredGrid const & grid, int degree) : topo_(grid.topology())
+{
+  if(degree < 2 || degree > 8)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "continuous levels of degree 2..8 (degree 1 is the Q1 level)");
+  q_       = make_cg_topology(topo_, degree);
+  n_nodes_ = (int64_t)q_.nn[0] * q_.nn[1] * q_.nn[2];
+  n_cells_ = grid.n_cells();
+  int const n = degree + 1;
+  n3_         = n * n * n;
+  ShapeInfo1D const sh = make_shape_info(degree);
+  // S, Dq, w, then S.S, SD.SD, S.SD with SD = Dq S the nodal derivative at the Gauss points
+  std::vector<double> SD((size_t)n * n, 0.0);
+  for(int q = 0; q < n; ++q)
+    for(int i = 0; i < n; ++i)
+      for(int m = 0; m < n; ++m)
+        SD[q * n + i] += sh.Dq[q * n + m] * sh.S[m * n + i];
+  std::vector<Number> tab;
+  for(double v : sh.S)
+    tab.push_back((Number)v);
+  for(double v : sh.Dq)
+    tab.push_back((Number)v);
+  for(double v : sh.qw)
+    tab.push_back((Number)v);
+  for(int e = 0; e < n * n; ++e)
+    tab.push_back((Number)(sh.S[e] * sh.S[e]));
+  for(int e = 0; e < n * n; ++e)
+    tab.push_back((Number)(SD[e] * SD[e]));
+  for(int e = 0; e < n * n; ++e)
+    tab.push_back((Number)(sh.S[e] * SD[e]));
+  tables_.upload(tab);
+  double const vol = grid.h(0) * grid.h(1) * grid.h(2);
+  for(int d = 0; d < 3; ++d)
+    hscale_[d] = (Number)(vol / (grid.h(d) * grid.h(d)));
+  if(!grid.affine())
+  {
+    GeometryData const g = compute_geometry(grid, sh, 1.0, true);
+    metric_.upload(std::vector<Number>(g.cell_metric.begin(), g.cell_metric.end()));
+  }
+  cell_values_.reinit(n_cells_ * n3_);
+}
+
+template<typename Number>
+CGCellArgs<Number> CGOperator<Number>::cell_args(Number const * src, bool diagonal) const
+{
+  CGCellArgs<Number> a;
+  a.q      = q_;
+  a.t      = topo_;
+  a.tables = tables_.data();
+  a.metric = metric_.size() ? metric_.data() : nullptr;
+  for(int d = 0; d < 3; ++d)
+    a.hscale[d] = hscale_[d];
+  a.src     = diagonal ? nullptr : src;
+  a.out     = cell_values_.data();
+  a.n_cells = n_cells_;
+  return a;
+}
+
+template<typename Number>
+void CGOperator<Number>::assemble(CGAssembleArgs<Number> a, CGAssembleMode mode, hipStream_t s) const
+{
+  a.q           = q_;
+  a.t           = topo_;
+  a.cell_values = cell_values_.data();
+  hipError_t e  = launch_cg_assemble<Number>(a, mode, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+}
+
+template<typename Number>
+void CGOperator<Number>::vmult(Number * dst, Number const * src, hipStream_t s) const
+{
+  hipError_t e = launch_cg_cell<Number>(cell_args(src, false), false, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  CGAssembleArgs<Number> a;
+  a.dst = dst;
+  a.x   = src;
+  assemble(a, CGAssembleMode::vmult, s);
+}
+
+template<typename Number>
+void CGOperator<Number>::residual(Number * dst, Number const * src, Number const * b,
+                                  hipStream_t s) const
+{
+  hipError_t e = launch_cg_cell<Number>(cell_args(src, false), false, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  CGAssembleArgs<Number> a;
+  a.dst = dst;
+  a.x   = src;
+  a.b   = b;
+  assemble(a, CGAssembleMode::residual, s);
+}
+
+template<typename Number>
+bool CGOperator<Number>::chebyshev_step(Number * x_new, Number const * x, Number const * x_old,
+                                        Number const * b, Number const * dinv, Number f1, Number f2,
+                                        bool has_old, hipStream_t s) const
+{
+  hipError_t e = launch_cg_cell<Number>(cell_args(x, false), false, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  CGAssembleArgs<Number> a;
+  a.dst     = x_new;
+  a.x       = x;
+  a.b       = b;
+  a.x_old   = x_old;
+  a.dinv    = dinv;
+  a.f1      = f1;
+  a.f2      = f2;
+  a.has_old = has_old ? 1 : 0;
+  assemble(a, CGAssembleMode::chebyshev, s);
+  return true;
+}
+
+template<typename Number>
+void CGOperator<Number>::inverse_diagonal(Number * d, hipStream_t s) const
+{
+  hipError_t e = launch_cg_cell<Number>(cell_args(nullptr, true), true, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  CGAssembleArgs<Number> a;
+  a.dst = d;
+  assemble(a, CGAssembleMode::diagonal, s);
+  Poisson::invert_diagonal_device<Number>(d, n_nodes_, s);
+}
+
 // ---------------------------------------------------------------- Chebyshev
 
 template<typename Number>
@@ -369,21 +501,19 @@ void ChebyshevSmoother<Number>::setup(LevelOperator<Number> const & op, int degr
   inv_diag_.reinit(n);
   t_.reinit(n);
   op.inverse_diagonal(inv_diag_.data(), s);
-  // start vector of deal.II's eigenvalue estimate: (i mod 11) minus its mean, constrained
-  // entries zero; eig_cg_iterations steps of Jacobi-PCG, Lanczos tridiagonal
+  // start vector of deal.II's eigenvalue estimate: (i mod 11) minus its mean (the smoother
+  // carries no constraints object, so constrained entries are not zeroed);
+  // eig_cg_iterations steps of Jacobi-PCG, Lanczos tridiagonal
   std::vector<Number> h((size_t)n);
   double              mean = 0;
   for(int64_t i = 0; i < n; ++i)
     mean += (double)(i % 11);
   mean /= (double)n;
-  std::vector<char> const mask = op.constrained_mask();
-  double                  nrm2 = 0;
+  double nrm2 = 0;
   for(int64_t i = 0; i < n; ++i)
   {
-    double v = (double)(i % 11) - mean;
-    if(!mask.empty() && mask[i])
-      v = 0;
-    h[i] = (Number)v;
+    double const v = (double)(i % 11) - mean;
+    h[i]           = (Number)v;
     nrm2 += v * v;
   }
   DeviceVector<Number> b(n), x(n);
@@ -502,35 +632,232 @@ void ChebyshevSmoother<Number>::apply_step(Number * x_new, Number const * x, Num
 
 // ---------------------------------------------------------------- multigrid
 
+std::vector<MGLevelInfo> multigrid_levels(MultigridType type, std::vector<int> const & degrees,
+                                          int n_h_levels, bool coarse_q1)
+{
+  using T = MultigridType;
+  if(degrees.empty())
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "multigrid needs a p-sequence");
+  for(size_t i = 1; i < degrees.size(); ++i)
+    if(degrees[i] >= degrees[i - 1] || degrees[i] < 1)
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "degrees must decrease strictly from fine to coarse");
+  if(type == T::legacy)
+  {
+    if(coarse_q1 && degrees.back() != 1)
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "the continuous Q1 level must follow a DG degree-1 level");
+    type = coarse_q1 ? T::pcMG : T::pMG;
+  }
+  // which transfers the type involves, and where the c-transfer sits relative to p
+  char const * const name_of[] = {"", "h", "ch", "hc", "c", "p", "cp", "pc", "hp", "chp", "hcp",
+                                  "hpc", "ph", "cph", "pch", "phc"};
+  int const ti = (int)type;
+  if(ti < 1 || ti > 15)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "unknown multigrid type");
+  std::string const name = name_of[ti];
+  bool const has_h = name.find('h') != std::string::npos;
+  bool const has_p = name.find('p') != std::string::npos;
+  bool const has_c = name.find('c') != std::string::npos;
+  int const  H     = has_h ? std::max(1, n_h_levels) : 1;
+  int const  k     = degrees[0];
+  // p-levels coarse to fine (the mesh is filled in below)
+  std::vector<MGLevelInfo> P;
+  if(!has_p)
+  {
+    if(has_c)
+      P.push_back({0, k, false});
+    P.push_back({0, k, true});
+  }
+  else
+  {
+    bool const c_first = has_c && name.find('c') < name.find('p');
+    bool const c_last  = has_c && !c_first;
+    if(c_last)
+      P.push_back({0, degrees.back(), false});
+    for(auto it = degrees.rbegin(); it != degrees.rend(); ++it)
+      P.push_back({0, *it, !c_first});
+    if(c_first)
+      P.push_back({0, k, true});
+  }
+  int const                np = (int)P.size();
+  std::vector<MGLevelInfo> L;
+  auto on = [&](int h, MGLevelInfo p) { L.push_back({h, p.degree, p.dg}); };
+  switch(type)
+  {
+    case T::hMG:
+      for(int h = 0; h < H; ++h)
+        on(h, P[0]);
+      break;
+    case T::cMG:
+    case T::pMG:
+    case T::cpMG:
+    case T::pcMG:
+      for(auto const & p : P)
+        on(H - 1, p);
+      break;
+    case T::chMG:
+      for(int h = 0; h < H; ++h)
+        on(h, P[0]);
+      on(H - 1, P[1]);
+      break;
+    case T::hcMG:
+      on(0, P[0]);
+      for(int h = 0; h < H; ++h)
+        on(h, P[1]);
+      break;
+    case T::phMG:
+    case T::cphMG:
+    case T::pchMG:
+      for(int h = 0; h + 1 < H; ++h)
+        on(h, P[0]);
+      for(auto const & p : P)
+        on(H - 1, p);
+      break;
+    case T::hpMG:
+    case T::hcpMG:
+    case T::hpcMG:
+      for(int i = 0; i + 1 < np; ++i)
+        on(0, P[i]);
+      for(int h = 0; h < H; ++h)
+        on(h, P[np - 1]);
+      break;
+    case T::phcMG:
+      on(0, P[0]);
+      for(int h = 0; h + 1 < H; ++h)
+        on(h, P[1]);
+      for(int i = 1; i < np; ++i)
+        on(H - 1, P[i]);
+      break;
+    case T::chpMG:
+      for(int i = 0; i + 2 < np; ++i)
+        on(0, P[i]);
+      for(int h = 0; h < H; ++h)
+        on(h, P[np - 2]);
+      on(H - 1, P[np - 1]);
+      break;
+    default:
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "unknown multigrid type");
+  }
+  for(size_t l = 1; l < L.size(); ++l)
+    if((L[l].h != L[l - 1].h) + (L[l].degree != L[l - 1].degree) + (L[l].dg != L[l - 1].dg) != 1)
+      EXADG_THROW(EXADG_ERR_ARGUMENT,
+                  "between two multigrid levels exactly one of mesh, degree, continuity may change");
+  return L;
+}
+
+namespace
+{
+// 1D continuous transfer from (nc_c cells, degree pc) to (nc_f cells, degree pf) on GL nodes,
+// nc_f = nc_c (p) or 2 nc_c (h): row I of the prolongation interpolates the coarse cell that
+// contains fine node I; entries of magnitude below 1e-14 (coinciding nodes) are dropped
+struct Line1D
+{
+  int                 rows = 0, width = 0;
+  std::vector<int>    idx;
+  std::vector<double> w;
+};
+
+Line1D line_prolongation(int nc_c, int pc, int nc_f, int pf, bool periodic)
+{
+  Line1D L;
+  int const rows_c = nc_c * pc + (periodic ? 0 : 1);
+  L.rows           = nc_f * pf + (periodic ? 0 : 1);
+  L.width          = pc + 1;
+  std::vector<double> xf, xc;
+  gauss_lobatto_points(pf + 1, xf);
+  gauss_lobatto_points(pc + 1, xc);
+  std::vector<double> val(pc + 1);
+  L.idx.assign((size_t)L.rows * L.width, 0);
+  L.w.assign((size_t)L.rows * L.width, 0.0);
+  for(int I = 0; I < L.rows; ++I)
+  {
+    int const    cf = std::min(I / pf, nc_f - 1), a = I - cf * pf;
+    int const    parent = nc_f == nc_c ? cf : cf / 2;
+    double const xi     = nc_f == nc_c ? xf[a] : 0.5 * ((cf % 2) + xf[a]);
+    lagrange_basis(xc, xi, val.data(), nullptr);
+    for(int j = 0; j <= pc; ++j)
+    {
+      int const c                  = parent * pc + j;
+      L.idx[(size_t)I * L.width + j] = periodic ? c % rows_c : c;
+      L.w[(size_t)I * L.width + j]   = std::fabs(val[j]) < 1e-14 ? 0.0 : val[j];
+    }
+  }
+  return L;
+}
+
+Line1D line_transpose(Line1D const & P, int rows_c)
+{
+  std::vector<std::vector<std::pair<int, double>>> col((size_t)rows_c);
+  for(int I = 0; I < P.rows; ++I)
+    for(int j = 0; j < P.width; ++j)
+    {
+      double const w = P.w[(size_t)I * P.width + j];
+      if(w != 0.0)
+        col[P.idx[(size_t)I * P.width + j]].push_back({I, w});
+    }
+  Line1D R;
+  R.rows  = rows_c;
+  R.width = 1;
+  for(auto const & c : col)
+    R.width = std::max(R.width, (int)c.size());
+  R.idx.assign((size_t)R.rows * R.width, 0);
+  R.w.assign((size_t)R.rows * R.width, 0.0);
+  for(int a = 0; a < rows_c; ++a)
+    for(size_t j = 0; j < col[a].size(); ++j)
+    {
+      R.idx[(size_t)a * R.width + j] = col[a][j].first;
+      R.w[(size_t)a * R.width + j]   = col[a][j].second;
+    }
+  return R;
+}
+
+int automatic_h_levels(StructuredGrid const & g)
+{
+  int H = 1;
+  for(;;)
+  {
+    bool ok = true;
+    for(int d = 0; d < 3; ++d)
+    {
+      int const n = g.mesh().n_cells[d];
+      if(n % (1 << H) != 0 || (n >> H) < 2)
+        ok = false;
+    }
+    if(!ok)
+      return H;
+    ++H;
+  }
+}
+}  // namespace
+
 template<typename Number>
 MultigridPreconditioner<Number>::MultigridPreconditioner(std::shared_ptr<StructuredGrid const> grid,
                                                          MultigridData const & data, double ip,
                                                          hipStream_t s)
   : grid_(std::move(grid)), data_(data)
 {
-  if(data.degrees.empty())
-    EXADG_THROW(EXADG_ERR_ARGUMENT, "multigrid needs at least one DG level");
-  for(size_t i = 1; i < data.degrees.size(); ++i)
-    if(data.degrees[i] >= data.degrees[i - 1] || data.degrees[i] < 1)
-      EXADG_THROW(EXADG_ERR_ARGUMENT, "DG degrees must decrease strictly from fine to coarse");
-  if(data.coarse_q1 && data.degrees.back() != 1)
-    EXADG_THROW(EXADG_ERR_ARGUMENT, "the continuous Q1 level must follow a DG degree-1 level");
-  // levels coarse (0) to fine
-  int const n_levels = (int)data.degrees.size() + (data.coarse_q1 ? 1 : 0);
+  int const n_h = data.n_h_levels > 0 ? data.n_h_levels : automatic_h_levels(*grid_);
+  std::vector<MGLevelInfo> const info =
+    multigrid_levels(data.type, data.degrees, n_h, data.coarse_q1);
+  int const n_levels = (int)info.size();
+  int const H        = info.back().h + 1;
+  std::vector<std::shared_ptr<StructuredGrid const>> grids((size_t)H);
+  grids[H - 1] = grid_;
+  for(int h = 0; h + 1 < H; ++h)
+    grids[h] = std::make_shared<StructuredGrid const>(grid_->coarsened(H - 1 - h));
   levels_.resize(n_levels);
-  int l = 0;
-  if(data.coarse_q1)
+  for(int l = 0; l < n_levels; ++l)
   {
-    levels_[0].op     = std::make_unique<Q1Operator<Number>>(*grid_);
-    levels_[0].degree = 1;
-    levels_[0].dg     = false;
-    ++l;
-  }
-  for(int i = (int)data.degrees.size() - 1; i >= 0; --i, ++l)
-  {
-    levels_[l].op     = std::make_unique<DGLevelOperator<Number>>(grid_, data.degrees[i], ip);
-    levels_[l].degree = data.degrees[i];
-    levels_[l].dg     = true;
+    Level & L = levels_[l];
+    L.h       = info[l].h;
+    L.degree  = info[l].degree;
+    L.dg      = info[l].dg;
+    L.grid    = grids[L.h];
+    if(L.dg)
+      L.op = std::make_unique<DGLevelOperator<Number>>(L.grid, L.degree, ip);
+    else if(L.degree == 1)
+      L.op = std::make_unique<Q1Operator<Number>>(*L.grid);
+    else
+      L.op = std::make_unique<CGOperator<Number>>(*L.grid, L.degree);
   }
   for(int k = 0; k < n_levels; ++k)
   {
@@ -544,10 +871,21 @@ MultigridPreconditioner<Number>::MultigridPreconditioner(std::shared_ptr<Structu
     L.sol_alt = L.sol_b.data();
     vec::set_zero(L.sol, n, s);
     vec::set_zero(L.sol_alt, n, s);
-    if(k > 0 && L.dg && levels_[k - 1].dg)
+    if(k == 0)
+      continue;
+    Level const & C = levels_[k - 1];
+    if(C.dg != L.dg)
     {
-      // p-transfer: P[a][i] = phi^coarse_i(x^fine_a) on Gauss-Lobatto nodes
-      int const           nf = L.degree + 1, nc = levels_[k - 1].degree + 1;
+      if(C.dg)
+        EXADG_THROW(EXADG_ERR_ARGUMENT, "a c-transfer needs the continuous level below the DG one");
+      L.transfer = 'c';
+      continue;
+    }
+    L.transfer = C.degree != L.degree ? 'p' : 'h';
+    if(L.dg && L.transfer == 'p')
+    {
+      // P[a][i] = phi^coarse_i(x^fine_a) on Gauss-Lobatto nodes
+      int const           nf = L.degree + 1, nc = C.degree + 1;
       std::vector<double> xf, xc;
       gauss_lobatto_points(nf, xf);
       gauss_lobatto_points(nc, xc);
@@ -560,6 +898,56 @@ MultigridPreconditioner<Number>::MultigridPreconditioner(std::shared_ptr<Structu
           P[a * nc + i] = (Number)val[i];
       }
       L.P.upload(P);
+    }
+    else if(L.dg)
+    {
+      // child b of a parent: Ph[b][a][i] = phi_i((b + x_a) / 2)
+      int const           n = L.degree + 1;
+      std::vector<double> x, val(n);
+      gauss_lobatto_points(n, x);
+      std::vector<Number> P((size_t)2 * n * n);
+      for(int b = 0; b < 2; ++b)
+        for(int a = 0; a < n; ++a)
+        {
+          lagrange_basis(x, 0.5 * (b + x[a]), val.data(), nullptr);
+          for(int i = 0; i < n; ++i)
+            P[(size_t)b * n * n + a * n + i] = (Number)val[i];
+        }
+      L.P.upload(P);
+    }
+    else
+    {
+      CGTopology const & qf = *L.op->cg_topology();
+      CGTopology const & qc = *C.op->cg_topology();
+      std::vector<int>    idx;
+      std::vector<double> w;
+      Line1D              lines[6];
+      for(int d = 0; d < 3; ++d)
+      {
+        lines[d] = line_prolongation(qc.nc[d], qc.p, qf.nc[d], qf.p, qf.periodic[d] != 0);
+        lines[3 + d] = line_transpose(lines[d], qc.nn[d]);
+      }
+      std::vector<size_t> off(6);
+      for(int j = 0; j < 6; ++j)
+      {
+        off[j] = idx.size();
+        idx.insert(idx.end(), lines[j].idx.begin(), lines[j].idx.end());
+        w.insert(w.end(), lines[j].w.begin(), lines[j].w.end());
+      }
+      L.line_idx.upload(idx);
+      L.line_w.upload(std::vector<Number>(w.begin(), w.end()));
+      for(int j = 0; j < 6; ++j)
+      {
+        LineOperator<Number> & o = j < 3 ? L.line_prolong[j] : L.line_restrict[j - 3];
+        o.rows                   = lines[j].rows;
+        o.width                  = lines[j].width;
+        o.idx                    = L.line_idx.data() + off[j];
+        o.w                      = L.line_w.data() + off[j];
+      }
+      int64_t const f0 = qf.nn[0], f1 = qf.nn[1], f2 = qf.nn[2];
+      int64_t const c0 = qc.nn[0], c1 = qc.nn[1], c2 = qc.nn[2];
+      L.work_a.reinit(std::max(f0 * c1 * c2, c0 * f1 * f2));
+      L.work_b.reinit(std::max(f0 * f1 * c2, c0 * c1 * f2));
     }
   }
   int64_t const n0 = levels_[0].op->n();
@@ -606,13 +994,31 @@ void MultigridPreconditioner<Number>::prolongate_and_add(int l, Number * dst, Nu
 {
   Level const & F = levels_[l];
   Level const & C = levels_[l - 1];
-  hipError_t    e;
-  if(C.dg)
+  hipError_t    e = hipSuccess;
+  if(F.transfer == 'c')
+    e = launch_c_prolongate_add<Number>(dst, src, F.grid->topology(), *C.op->cg_topology(), s);
+  else if(F.dg && F.transfer == 'p')
     e = launch_p_prolongate_add<Number>(dst, src, F.P.data(), F.degree + 1, C.degree + 1,
-                                        grid_->n_cells(), s);
+                                        F.grid->n_cells(), s);
+  else if(F.dg)
+    e = launch_h_prolongate_dg_add<Number>(dst, src, F.P.data(), F.degree + 1, F.grid->topology(),
+                                           C.grid->topology(), s);
   else
-    e = launch_c_prolongate_add<Number>(dst, src, grid_->topology(),
-                                        static_cast<Q1Operator<Number> const &>(*C.op).q1_topology(), s);
+  {
+    // separable passes x, y, z: coarse -> (f0,c1,c2) -> (f0,f1,c2) -> fine
+    CGTopology const & qf = *F.op->cg_topology();
+    CGTopology const & qc = *C.op->cg_topology();
+    int const          d0[3] = {qc.nn[0], qc.nn[1], qc.nn[2]};
+    int const          d1[3] = {qf.nn[0], qc.nn[1], qc.nn[2]};
+    int const          d2[3] = {qf.nn[0], qf.nn[1], qc.nn[2]};
+    Number *           wa = const_cast<Number *>(F.work_a.data());
+    Number *           wb = const_cast<Number *>(F.work_b.data());
+    e = launch_line_apply<Number>(wa, src, d0, 0, F.line_prolong[0], &qc, nullptr, false, s);
+    if(e == hipSuccess)
+      e = launch_line_apply<Number>(wb, wa, d1, 1, F.line_prolong[1], nullptr, nullptr, false, s);
+    if(e == hipSuccess)
+      e = launch_line_apply<Number>(dst, wb, d2, 2, F.line_prolong[2], nullptr, &qf, true, s);
+  }
   if(e != hipSuccess)
     EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
 }
@@ -623,13 +1029,31 @@ void MultigridPreconditioner<Number>::restrict_and_add(int l, Number * dst, Numb
 {
   Level const & F = levels_[l];
   Level const & C = levels_[l - 1];
-  hipError_t    e;
-  if(C.dg)
+  hipError_t    e = hipSuccess;
+  if(F.transfer == 'c')
+    e = launch_c_restrict_add<Number>(dst, src, F.grid->topology(), *C.op->cg_topology(), s);
+  else if(F.dg && F.transfer == 'p')
     e = launch_p_restrict_add<Number>(dst, src, F.P.data(), F.degree + 1, C.degree + 1,
-                                      grid_->n_cells(), s);
+                                      F.grid->n_cells(), s);
+  else if(F.dg)
+    e = launch_h_restrict_dg_add<Number>(dst, src, F.P.data(), F.degree + 1, F.grid->topology(),
+                                         C.grid->topology(), s);
   else
-    e = launch_c_restrict_add<Number>(dst, src, grid_->topology(),
-                                      static_cast<Q1Operator<Number> const &>(*C.op).q1_topology(), s);
+  {
+    // transposed passes x, y, z: fine -> (c0,f1,f2) -> (c0,c1,f2) -> coarse
+    CGTopology const & qf = *F.op->cg_topology();
+    CGTopology const & qc = *C.op->cg_topology();
+    int const          d0[3] = {qf.nn[0], qf.nn[1], qf.nn[2]};
+    int const          d1[3] = {qc.nn[0], qf.nn[1], qf.nn[2]};
+    int const          d2[3] = {qc.nn[0], qc.nn[1], qf.nn[2]};
+    Number *           wa = const_cast<Number *>(F.work_a.data());
+    Number *           wb = const_cast<Number *>(F.work_b.data());
+    e = launch_line_apply<Number>(wa, src, d0, 0, F.line_restrict[0], &qf, nullptr, false, s);
+    if(e == hipSuccess)
+      e = launch_line_apply<Number>(wb, wa, d1, 1, F.line_restrict[1], nullptr, nullptr, false, s);
+    if(e == hipSuccess)
+      e = launch_line_apply<Number>(dst, wb, d2, 2, F.line_restrict[2], nullptr, &qc, true, s);
+  }
   if(e != hipSuccess)
     EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
 }
@@ -775,6 +1199,8 @@ template class LevelOperator<double>;
 template class LevelOperator<float>;
 template class Q1Operator<double>;
 template class Q1Operator<float>;
+template class CGOperator<double>;
+template class CGOperator<float>;
 template class ChebyshevSmoother<double>;
 template class ChebyshevSmoother<float>;
 template class MultigridPreconditioner<double>;

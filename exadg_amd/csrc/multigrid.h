@@ -1,5 +1,7 @@
-// Hybrid p-multigrid preconditioner (DG_k -> ... -> DG_1 -> continuous Q1) with
-// Chebyshev-Jacobi smoothing, run in FP32 or FP64 inside an FP64 outer Krylov solver.
+// Hybrid multigrid preconditioner over h-levels (2x-coarsened meshes), p-levels (lower
+// degrees) and c-levels (continuous FE_Q), in any of the reference's MultigridType orders
+// (default: DG_k -> ... -> DG_1 -> continuous Q1), with Chebyshev-Jacobi smoothing, run in FP32
+// or FP64 inside an FP64 outer Krylov solver.
 //
 // Reference interfaces: MultigridPreconditionerBase::vmult/update
 // (solvers_and_preconditioners/multigrid/multigrid_preconditioner_base.cpp:55-95,673-683),
@@ -12,6 +14,7 @@
 #include <memory>
 #include <vector>
 
+#include "cg_kernels.h"
 #include "common.h"
 #include "grid.h"
 #include "laplace_operator.h"
@@ -55,10 +58,36 @@ enum class CoarseSolver : int
   chebyshev = 1   // MGCoarseChebyshev: Chebyshev-Jacobi with degree from the tolerance
 };
 
+// MultigridType (multigrid_parameters.h:39-57), numbered as there; legacy = pcMG when
+// coarse_q1 is set, else pMG
+enum class MultigridType : int
+{
+  legacy = 0, hMG, chMG, hcMG, cMG, pMG, cpMG, pcMG, hpMG, chpMG, hcpMG, hpcMG, phMG, cphMG, pchMG,
+  phcMG
+};
+
+// one multigrid level: mesh (h = 0 coarsest), degree, discontinuous or continuous
+struct MGLevelInfo
+{
+  int  h      = 0;
+  int  degree = 1;
+  bool dg     = true;
+};
+
+// the level list of MultigridPreconditionerBase::initialize_levels
+// (multigrid_preconditioner_base.cpp:97-323), coarse to fine, from the type, the p-sequence
+// (degrees fine to coarse) and the number of meshes of the h-levels
+std::vector<MGLevelInfo> multigrid_levels(MultigridType type, std::vector<int> const & degrees,
+                                          int n_h_levels, bool coarse_q1);
+
 struct MultigridData
 {
-  std::vector<int> degrees = {3, 1};     // DG degrees, fine to coarse (degrees[0] = k)
-  bool             coarse_q1 = true;     // append a continuous Q1 level via a c-transfer
+  std::vector<int> degrees = {3, 1};     // p-sequence, fine to coarse (degrees[0] = k)
+  bool             coarse_q1 = true;     // legacy type: append a continuous Q1 level
+  MultigridType    type      = MultigridType::legacy;
+  // meshes of the h-levels, the finest included (types with h-coarsening); 0: halve while
+  // every cells-per-direction count is even and the result keeps at least 2
+  int              n_h_levels = 0;
   int              smoother_degree = 5;  // SmootherData::iterations
   double           smoothing_range = 20; // SmootherData::smoothing_range
   int              eig_cg_iterations = 20;
@@ -94,6 +123,8 @@ public:
   virtual void    vmult(Number * dst, Number const * src, hipStream_t s) const = 0;
   virtual void    inverse_diagonal(Number * d, hipStream_t s) const           = 0;
   virtual bool    continuous() const { return false; }
+  // node numbering of a continuous level (null for DG levels)
+  virtual CGTopology const * cg_topology() const { return nullptr; }
   // dst = b - A src; operators with a fused epilogue override this (the default applies A and
   // then updates, so it needs dst != b)
   virtual void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const;
@@ -124,15 +155,48 @@ public:
                       Number const * dinv, Number f1, Number f2, bool has_old,
                       hipStream_t s) const override;
   Q1Topology const & q1_topology() const { return q_; }
+  CGTopology const * cg_topology() const override { return &cg_; }
   std::vector<double> const & element_matrices() const { return mats_host_; }
 
 private:
   BoxTopology          topo_;
   Q1Topology           q_;
+  CGTopology           cg_;
   int64_t              n_nodes_;
   std::vector<double>  mats_host_;  // 8x8 element matrices (one shared one if affine)
   DeviceBuffer<Number> stencil_;    // assembled 27-point stencil, 27 planes of n_nodes
   DeviceBuffer<Number> diag_;       // sum over cells of K_vv (1 on constrained rows)
+};
+
+// continuous Q_p Laplace (p >= 2) on the structured box: the cell integrals of the SIPG form
+// (its face terms vanish on continuous functions), summed over the cells sharing a node;
+// Dirichlet nodes read as zero and act as identity rows (operator_base.cpp:299-309)
+template<typename Number>
+class CGOperator : public LevelOperator<Number>
+{
+public:
+  CGOperator(StructuredGrid const & grid, int degree);
+  int64_t n() const override { return n_nodes_; }
+  void    vmult(Number * dst, Number const * src, hipStream_t s) const override;
+  void    inverse_diagonal(Number * d, hipStream_t s) const override;
+  void    residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const override;
+  bool    continuous() const override { return true; }
+  CGTopology const * cg_topology() const override { return &q_; }
+  bool chebyshev_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
+                      Number const * dinv, Number f1, Number f2, bool has_old,
+                      hipStream_t s) const override;
+
+private:
+  CGCellArgs<Number> cell_args(Number const * src, bool diagonal) const;
+  void               assemble(CGAssembleArgs<Number> a, CGAssembleMode mode, hipStream_t s) const;
+
+  BoxTopology                  topo_;
+  CGTopology                   q_;
+  int64_t                      n_nodes_ = 0, n_cells_ = 0;
+  int                          n3_      = 0;
+  Number                       hscale_[3];
+  DeviceBuffer<Number>         tables_, metric_;
+  mutable DeviceVector<Number> cell_values_;
 };
 
 // Chebyshev polynomial smoother with point-Jacobi inner preconditioner (deal.II
@@ -182,6 +246,7 @@ public:
   bool vmult_allocates() const override { return uses_graph(); }
 
   int n_levels() const { return (int)levels_.size(); }
+  MGLevelInfo level_info(int l) const { return {levels_[l].h, levels_[l].degree, levels_[l].dg}; }
   // level l: 0 = coarsest
   LevelOperator<Number> const & level_operator(int l) const { return *levels_[l].op; }
   ChebyshevSmoother<Number> const & smoother(int l) const { return levels_[l].smoother; }
@@ -195,12 +260,23 @@ private:
   struct Level
   {
     std::unique_ptr<LevelOperator<Number>> op;
+    int                                    h      = 0;
     int                                    degree = 1;
     bool                                   dg     = true;
+    std::shared_ptr<StructuredGrid const>  grid;
     DeviceVector<Number>                   defect, sol_a, sol_b, t;
     Number *                               sol = nullptr, *sol_alt = nullptr;
     ChebyshevSmoother<Number>              smoother;
-    DeviceBuffer<Number>                   P;  // p-transfer matrix to the next coarser level
+    // transfer to the next coarser level: 'p', 'h' or 'c'
+    char                                   transfer = 0;
+    // DG: p-transfer matrix (n_fine x n_coarse) or the two child matrices of the h-transfer
+    DeviceBuffer<Number>                   P;
+    // continuous p/h-transfer: per direction a prolongation and a restriction line operator
+    // (indices and weights), and two work arrays for the separable passes
+    DeviceBuffer<int>                      line_idx;
+    DeviceBuffer<Number>                   line_w;
+    LineOperator<Number>                   line_prolong[3], line_restrict[3];
+    DeviceVector<Number>                   work_a, work_b;
   };
   void v_cycle(int l, hipStream_t s);
   void coarse_solve(hipStream_t s);

@@ -150,10 +150,36 @@ enum exadg_coarse_solver
   EXADG_COARSE_CHEBYSHEV = 1  /* MGCoarseChebyshev: degree chosen for coarse_rel_tol        */
 };
 
-/* MultigridData (solvers_and_preconditioners/multigrid/multigrid_parameters.h:39-313) for the
- * hybrid p-multigrid DG_k -> ... -> DG_1 [-> continuous Q1]: the DG degrees are listed fine to
- * coarse (dg_degrees[0] = operator degree, strictly decreasing; the last must be 1 when
- * coarse_q1 = 1). precision is the V-cycle's floating-point type (MultigridNumber). */
+/* MultigridType (multigrid_parameters.h:39-57), numbered as there. EXADG_MG_LEGACY selects
+ * pcMG when coarse_q1 = 1 and pMG otherwise. */
+enum exadg_mg_type
+{
+  EXADG_MG_LEGACY = 0,
+  EXADG_MG_HMG    = 1,
+  EXADG_MG_CHMG   = 2,
+  EXADG_MG_HCMG   = 3,
+  EXADG_MG_CMG    = 4,
+  EXADG_MG_PMG    = 5,
+  EXADG_MG_CPMG   = 6,
+  EXADG_MG_PCMG   = 7,
+  EXADG_MG_HPMG   = 8,
+  EXADG_MG_CHPMG  = 9,
+  EXADG_MG_HCPMG  = 10,
+  EXADG_MG_HPCMG  = 11,
+  EXADG_MG_PHMG   = 12,
+  EXADG_MG_CPHMG  = 13,
+  EXADG_MG_PCHMG  = 14,
+  EXADG_MG_PHCMG  = 15
+};
+
+/* MultigridData (solvers_and_preconditioners/multigrid/multigrid_parameters.h:39-313) of the
+ * hybrid multigrid. dg_degrees is the p-sequence, fine to coarse (dg_degrees[0] = operator
+ * degree, strictly decreasing); mg_type orders the h-, p- and c-levels as
+ * MultigridPreconditionerBase::initialize_levels does (multigrid_preconditioner_base.cpp:97-323):
+ * c-levels are continuous FE_Q of the same degree, h-levels the box with every cell count
+ * halved (n_h_levels meshes including the finest; 0 = halve while every count is even and keeps
+ * at least 2 cells). With EXADG_MG_LEGACY the last degree must be 1 when coarse_q1 = 1.
+ * precision is the V-cycle's floating-point type (MultigridNumber). */
 typedef struct exadg_mg_desc
 {
   int32_t n_dg_levels;
@@ -170,6 +196,8 @@ typedef struct exadg_mg_desc
   int32_t capture_graph;     /* replay the V-cycle as a HIP graph when it has no host
                                 synchronization (Chebyshev coarse solver); default 0: on
                                 MI355X the replay measured no faster than eager launches */
+  int32_t mg_type;           /* exadg_mg_type (default EXADG_MG_LEGACY)                */
+  int32_t n_h_levels;        /* meshes of the h-levels, 0 = automatic (default)        */
 } exadg_mg_desc;
 
 /* Defaults of the reference for degree k: PSequenceType::Bisect (p -> max(p/2,1),
@@ -197,6 +225,11 @@ int exadg_preconditioner_vmult(exadg_preconditioner * p, double * dst, const dou
  * reference: it polishes an initial guess (the learned one) before the CG solve. */
 int exadg_preconditioner_smooth(exadg_laplace * op, exadg_preconditioner * p, const double * b,
                                 double * x, void * stream);
+/* The multigrid level list, coarse (0) to fine: mesh index h (0 = coarsest mesh), degree,
+ * discontinuous (1) or continuous (0), and size; arrays of at least max_levels entries (any
+ * may be NULL); *n_levels receives the count. */
+int exadg_preconditioner_mg_levels(const exadg_preconditioner * p, int max_levels, int * n_levels,
+                                   int * h, int * degree, int * dg, int64_t * n);
 /* PreconditionerBase::update: recompute diagonals / eigenvalue estimates. */
 int exadg_preconditioner_update(exadg_preconditioner * p, void * stream);
 int exadg_preconditioner_destroy(exadg_preconditioner * p);
