@@ -59,6 +59,7 @@ class MGDesc(ctypes.Structure):
         ("capture_graph", ctypes.c_int32),
         ("mg_type", ctypes.c_int32),
         ("n_h_levels", ctypes.c_int32),
+        ("smoother_preconditioner", ctypes.c_int32),
     ]
 
 
@@ -80,12 +81,15 @@ class SolveInfo(ctypes.Structure):
 
 
 PRECOND_NONE, PRECOND_POINT_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
+PRECOND_BLOCK_JACOBI, PRECOND_ADDITIVE_SCHWARZ = 3, 4
+SMOOTHER_POINT_JACOBI, SMOOTHER_BLOCK_JACOBI, SMOOTHER_ADDITIVE_SCHWARZ = 0, 1, 2
 COARSE_CG, COARSE_CHEBYSHEV = 0, 1
 
 
 def mg_desc(degree, degrees=None, precision=FP32, coarse_q1=True, coarse_solver=COARSE_CG,
             coarse_rel_tol=1e-3, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=20,
-            singular=False, capture_graph=False, mg_type="legacy", n_h_levels=0):
+            singular=False, capture_graph=False, mg_type="legacy", n_h_levels=0,
+            smoother_preconditioner=0):
     """MultigridData; default p-sequence Bisect (k, k/2, ..., 1) + continuous Q1 level.
     mg_type: a name of MG_TYPES (or its index); n_h_levels: meshes of the h-levels (0 =
     automatic)."""
@@ -106,6 +110,7 @@ def mg_desc(degree, degrees=None, precision=FP32, coarse_q1=True, coarse_solver=
     d.capture_graph = int(capture_graph)
     d.mg_type = MG_TYPES.index(mg_type) if isinstance(mg_type, str) else int(mg_type)
     d.n_h_levels = int(n_h_levels)
+    d.smoother_preconditioner = int(smoother_preconditioner)
     return d
 
 

@@ -387,6 +387,12 @@ int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_des
       h->p = std::make_unique<ExaDG::JacobiPreconditioner>(A);
       h->p->update(s);
     }
+    else if(kind == EXADG_PRECOND_BLOCK_JACOBI || kind == EXADG_PRECOND_ADDITIVE_SCHWARZ)
+    {
+      // on the DG space the additive Schwarz preconditioner is the block-Jacobi inverse
+      h->p = std::make_unique<ExaDG::BlockJacobiPreconditioner>(A);
+      h->p->update(s);
+    }
     else if(kind == EXADG_PRECOND_MULTIGRID)
     {
       require(mg != nullptr, "multigrid preconditioner needs an exadg_mg_desc");
@@ -399,6 +405,11 @@ int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_des
       require(mg->n_h_levels >= 0 && mg->n_h_levels <= 16, "n_h_levels out of range");
       d.type                 = static_cast<ExaDG::MultigridType>(mg->mg_type);
       d.n_h_levels           = mg->n_h_levels;
+      require(mg->smoother_preconditioner >= EXADG_SMOOTHER_POINT_JACOBI &&
+                mg->smoother_preconditioner <= EXADG_SMOOTHER_ADDITIVE_SCHWARZ,
+              "unknown smoother_preconditioner");
+      d.smoother_preconditioner =
+        static_cast<ExaDG::SmootherPreconditioner>(mg->smoother_preconditioner);
       d.smoother_degree      = mg->smoother_degree;
       d.smoothing_range      = mg->smoothing_range;
       d.eig_cg_iterations    = mg->eig_cg_iterations;

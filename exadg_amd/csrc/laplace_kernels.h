@@ -60,6 +60,10 @@ struct SipgArgs
   BoxTopology    topo;
   int64_t        n_cells;
   int            probe;      // diagonal mode: local index of the unit vector
+  // diagonal mode: 0 stores (A_KK)_{probe,probe} at dst[cell n^3 + probe]; 1 stores the whole
+  // response A_KK e_probe as column `probe` of the cell's n^3 x n^3 block,
+  // dst[(cell n^3 + probe) n^3 + e] (the block is symmetric, so this is also row `probe`)
+  int            block = 0;
   int            tile_rows;  // traversal: y-rows per tile (one tile per XCD)
   // chebyshev and residual modes
   Number const * x_old   = nullptr;

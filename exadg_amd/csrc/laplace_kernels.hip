@@ -656,7 +656,10 @@ __global__ void __launch_bounds__(256)
   {
     if constexpr(mode == ApplyMode::diagonal)
     {
-      if(lt == 0)
+      if(a.block)
+        for(int e = lt; e < N3; e += N2)
+          a.dst[(cell * N3 + a.probe) * N3 + e] = A[e];
+      else if(lt == 0)
         a.dst[cell * N3 + a.probe] = A[a.probe];
     }
     else if constexpr(mode == ApplyMode::vmult_add)
@@ -968,7 +971,9 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
       load_line<N>(U + bz, L::PY, v);
 #pragma unroll
       for(int k = 0; k < N; ++k)
-        if(lt + k * N2 == a.probe)
+        if(a.block)
+          a.dst[((size_t)cell * N3 + a.probe) * N3 + lt + k * N2] = outp[k] + v[k];
+        else if(lt + k * N2 == a.probe)
           a.dst[(size_t)cell * N3 + a.probe] = outp[k] + v[k];
     }
   }

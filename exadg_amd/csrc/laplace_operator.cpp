@@ -1,5 +1,7 @@
 #include "laplace_operator.h"
 
+#include "block_kernels.h"
+
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -245,6 +247,30 @@ void LaplaceOperator<Number>::calculate_inverse_diagonal(Number * diag, hipStrea
 {
   calculate_diagonal(diag, s);
   invert_diagonal_device<Number>(diag, m(), s);
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::calculate_block_diagonal(Number * blocks, hipStream_t s) const
+{
+  if(blocks == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  for(int probe = 0; probe < n3_; ++probe)
+  {
+    SipgArgs<Number> a = make_args(blocks, nullptr, probe);
+    a.block            = 1;
+    launch(ApplyMode::diagonal, a, s);
+  }
+}
+
+template<typename Number>
+void LaplaceOperator<Number>::calculate_inverse_block_diagonal(Number * blocks, hipStream_t s) const
+{
+  if(n3_ > kMaxDenseBlock)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "block Jacobi holds blocks up to 125 x 125 (degree <= 4)");
+  calculate_block_diagonal(blocks, s);
+  hipError_t e = launch_block_invert<Number>(blocks, n3_, n_cells(), s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
 }
 
 template<typename Number>

@@ -75,6 +75,13 @@ public:
   // 608-646): exact diagonal by unit-vector probing of the cell-local block
   void calculate_diagonal(Number * diag, hipStream_t s) const;
   void calculate_inverse_diagonal(Number * diag, hipStream_t s) const;
+  // OperatorBase::add_block_diagonal_matrices through the cell-based loop
+  // (cell_based_loop_block_diagonal, operator_base.cpp:649-680): the n^3 x n^3 diagonal block
+  // A_KK of every cell (cell integral plus its faces' self-coupling), column j of cell c at
+  // blocks[(c n^3 + j) n^3 ...]; one unit-vector probe per local index
+  void calculate_block_diagonal(Number * blocks, hipStream_t s) const;
+  // the inverses of those blocks (exact; blocks of at most 125 x 125, i.e. degree <= 4)
+  void calculate_inverse_block_diagonal(Number * blocks, hipStream_t s) const;
   // x_new = (1 + f1) x - f1 x_old (has_old) + f2 dinv (b - A x), the operator application and
   // the Chebyshev-Jacobi update in one kernel (PreconditionChebyshev's inner step,
   // chebyshev_smoother.h:37-227); x_new may alias x_old, not x

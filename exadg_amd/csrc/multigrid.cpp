@@ -63,12 +63,31 @@ void cg_tridiagonal_extreme_eigenvalues(std::vector<double> const & alpha,
   lmax = kth(n - 1);
 }
 
+template<typename Number>
+void JacobiData<Number>::apply(Number * z, Number const * r, int64_t n, hipStream_t s) const
+{
+  if(blocks == nullptr)
+  {
+    vec::scale(z, inv_diag, r, n, s);
+    return;
+  }
+  BlockApplyArgs<Number> a;
+  a.blocks     = blocks;
+  a.b          = r;
+  a.dst        = z;
+  a.m          = m;
+  a.n_cells    = n / m;
+  hipError_t e = launch_block_apply<Number>(a, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+}
+
 namespace
 {
-// host-synchronous PCG with point Jacobi on a level operator, recording the CG coefficients
-// (setup-time eigenvalue estimates; coarse solves)
+// host-synchronous PCG with point or block Jacobi on a level operator, recording the CG
+// coefficients (setup-time eigenvalue estimates; coarse solves)
 template<typename Number>
-int level_pcg(LevelOperator<Number> const & op, Number const * inv_diag, Number const * b, Number * x,
+int level_pcg(LevelOperator<Number> const & op, JacobiData<Number> const & P, Number const * b, Number * x,
               double tol_abs, int max_iter, bool singular, Reducer & red, hipStream_t s,
               PcgWorkspace<Number> & ws, std::vector<double> * alphas = nullptr,
               std::vector<double> * betas = nullptr, bool x_is_zero = true)
@@ -89,7 +108,7 @@ int level_pcg(LevelOperator<Number> const & op, Number const * inv_diag, Number 
   }
   if(singular)
     vec::subtract_mean(red, r.data(), n, s);
-  vec::scale(z.data(), inv_diag, r.data(), n, s);
+  P.apply(z.data(), r.data(), n, s);
   vec::norm2_and_dot(red, 1, 2, r.data(), z.data(), n, s);
   double const * h   = red.fetch(s, 4);
   double         rho = h[2];
@@ -107,7 +126,7 @@ int level_pcg(LevelOperator<Number> const & op, Number const * inv_diag, Number 
     vec::axpby(r.data(), Number(-alpha), v.data(), Number(1), n, s);
     if(singular)
       vec::subtract_mean(red, r.data(), n, s);
-    vec::scale(z.data(), inv_diag, r.data(), n, s);
+    P.apply(z.data(), r.data(), n, s);
     vec::norm2_and_dot(red, 1, 2, r.data(), z.data(), n, s);
     h                        = red.fetch(s, 4);
     double const rn          = std::sqrt(h[1]);
@@ -148,6 +167,11 @@ public:
   {
     op_->calculate_inverse_diagonal(d, s);
   }
+  int  block_size() const override { return (int)(op_->m() / op_->n_cells()); }
+  void inverse_block_diagonal(Number * blocks, hipStream_t s) const override
+  {
+    op_->calculate_inverse_block_diagonal(blocks, s);
+  }
   void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const override
   {
     op_->residual(dst, src, b, s);
@@ -187,6 +211,29 @@ void JacobiPreconditioner::update(hipStream_t s) { op_.calculate_inverse_diagona
 void JacobiPreconditioner::vmult(double * dst, double const * src, hipStream_t s)
 {
   vec::scale(dst, inv_diag_.data(), src, inv_diag_.size(), s);
+}
+
+BlockJacobiPreconditioner::BlockJacobiPreconditioner(Poisson::LaplaceOperator<double> const & op)
+  : op_(op), m_((int)(op.m() / op.n_cells()))
+{
+  if(m_ > kMaxDenseBlock)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "block Jacobi holds blocks up to 125 x 125 (degree <= 4)");
+  blocks_.reinit(op.n_cells() * (int64_t)m_ * m_);
+}
+
+void BlockJacobiPreconditioner::update(hipStream_t s)
+{
+  op_.calculate_inverse_block_diagonal(blocks_.data(), s);
+}
+
+void BlockJacobiPreconditioner::vmult(double * dst, double const * src, hipStream_t s)
+{
+  if(dst == src)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "dst and src must not alias");
+  JacobiData<double> P;
+  P.blocks = blocks_.data();
+  P.m      = m_;
+  P.apply(dst, src, op_.m(), s);
 }
 
 // ---------------------------------------------------------------- Q1 operator
@@ -492,15 +539,46 @@ void CGOperator<Number>::inverse_diagonal(Number * d, hipStream_t s) const
 // ---------------------------------------------------------------- Chebyshev
 
 template<typename Number>
+JacobiData<Number> ChebyshevSmoother<Number>::jacobi_data() const
+{
+  JacobiData<Number> P;
+  if(block_m_ > 0)
+  {
+    P.blocks = blocks_.data();
+    P.m      = block_m_;
+  }
+  else
+    P.inv_diag = inv_diag_.data();
+  return P;
+}
+
+template<typename Number>
 void ChebyshevSmoother<Number>::setup(LevelOperator<Number> const & op, int degree_,
-                                      double smoothing_range, int eig_cg_iterations, hipStream_t s)
+                                      double smoothing_range, int eig_cg_iterations, hipStream_t s,
+                                      SmootherPreconditioner prec)
 {
   op_          = &op;
   degree       = degree_;
   int64_t const n = op.n();
-  inv_diag_.reinit(n);
   t_.reinit(n);
-  op.inverse_diagonal(inv_diag_.data(), s);
+  block_m_ = 0;
+  if(prec == SmootherPreconditioner::point_jacobi)
+  {
+    inv_diag_.reinit(n);
+    op.inverse_diagonal(inv_diag_.data(), s);
+  }
+  else
+  {
+    // PreconditionerSmoother::BlockJacobi; AdditiveSchwarz is the same inverse on DG levels
+    if(op.continuous())
+      EXADG_THROW(EXADG_ERR_ARGUMENT,
+                  prec == SmootherPreconditioner::block_jacobi
+                    ? "Block Jacobi only implemented for DG!"
+                    : "additive Schwarz on continuous levels is not implemented");
+    block_m_ = op.block_size();
+    blocks_.reinit(n / block_m_ * (int64_t)block_m_ * block_m_);
+    op.inverse_block_diagonal(blocks_.data(), s);
+  }
   // start vector of deal.II's eigenvalue estimate: (i mod 11) minus its mean (the smoother
   // carries no constraints object, so constrained entries are not zeroed);
   // eig_cg_iterations steps of Jacobi-PCG, Lanczos tridiagonal
@@ -521,7 +599,7 @@ void ChebyshevSmoother<Number>::setup(LevelOperator<Number> const & op, int degr
   Reducer              red;
   PcgWorkspace<Number> ws;
   std::vector<double>  al, be;
-  cg_iterations = level_pcg<Number>(op, inv_diag_.data(), b.data(), x.data(), 1e-2 * std::sqrt(nrm2),
+  cg_iterations = level_pcg<Number>(op, jacobi_data(), b.data(), x.data(), 1e-2 * std::sqrt(nrm2),
                                     eig_cg_iterations, false, red, s, ws, &al, &be);
   cg_tridiagonal_extreme_eigenvalues(al, be, lambda_min_estimate, lambda_max_estimate);
   // interval [lambda_max/range, lambda_max] with a 1.2 safety factor on the CG estimate
@@ -564,7 +642,7 @@ void ChebyshevSmoother<Number>::setup_coarse(LevelOperator<Number> const & op, d
   Reducer              red;
   PcgWorkspace<Number> ws;
   std::vector<double>  al, be;
-  cg_iterations = level_pcg<Number>(op, inv_diag_.data(), b.data(), x.data(), 1e-5 * std::sqrt(nrm2),
+  cg_iterations = level_pcg<Number>(op, jacobi_data(), b.data(), x.data(), 1e-5 * std::sqrt(nrm2),
                                     10000, singular, red, s, ws, &al, &be);
   cg_tridiagonal_extreme_eigenvalues(al, be, lambda_min_estimate, lambda_max_estimate);
   double const factor = 1.1;
@@ -584,8 +662,22 @@ void ChebyshevSmoother<Number>::vmult(Number *& x, Number *& x_alt, Number const
                                       hipStream_t s) const
 {
   int64_t const n = op_->n();
-  vec::chebyshev_update<Number>(x, nullptr, nullptr, b, nullptr, inv_diag_.data(), Number(0),
-                                Number(1. / theta), true, false, n, s);
+  if(block_m_ > 0)
+  {
+    BlockApplyArgs<Number> a;
+    a.blocks     = blocks_.data();
+    a.b          = b;
+    a.dst        = x;
+    a.f2         = Number(1. / theta);
+    a.m          = block_m_;
+    a.n_cells    = n / block_m_;
+    hipError_t e = launch_block_apply<Number>(a, s);
+    if(e != hipSuccess)
+      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  }
+  else
+    vec::chebyshev_update<Number>(x, nullptr, nullptr, b, nullptr, inv_diag_.data(), Number(0),
+                                  Number(1. / theta), true, false, n, s);
   if(degree < 2 || std::fabs(delta) < 1e-40)
     return;
   double rhok = delta / theta, sigma = theta / delta;
@@ -623,6 +715,27 @@ void ChebyshevSmoother<Number>::apply_step(Number * x_new, Number const * x, Num
                                            Number const * b, Number f1, Number f2, bool has_old,
                                            hipStream_t s) const
 {
+  if(block_m_ > 0)
+  {
+    // t = A x, then the block inverse with the Chebyshev update in its epilogue
+    op_->vmult(t_.data(), x, s);
+    BlockApplyArgs<Number> a;
+    a.blocks     = blocks_.data();
+    a.b          = b;
+    a.t          = t_.data();
+    a.x          = x;
+    a.x_old      = x_old;
+    a.dst        = x_new;
+    a.f1         = f1;
+    a.f2         = f2;
+    a.variant    = has_old ? 2 : 1;
+    a.m          = block_m_;
+    a.n_cells    = op_->n() / block_m_;
+    hipError_t e = launch_block_apply<Number>(a, s);
+    if(e != hipSuccess)
+      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+    return;
+  }
   if(op_->chebyshev_step(x_new, x, x_old, b, inv_diag_.data(), f1, f2, has_old, s))
     return;
   op_->vmult(t_.data(), x, s);
@@ -980,7 +1093,7 @@ void MultigridPreconditioner<Number>::update(hipStream_t s)
   clear_graphs();  // the captured kernels hold the Chebyshev coefficients as arguments
   for(int l = 1; l < n_levels(); ++l)
     levels_[l].smoother.setup(*levels_[l].op, data_.smoother_degree, data_.smoothing_range,
-                              data_.eig_cg_iterations, s);
+                              data_.eig_cg_iterations, s, data_.smoother_preconditioner);
   if(data_.coarse_solver == CoarseSolver::chebyshev)
     coarse_cheb_.setup_coarse(*levels_[0].op, data_.coarse_rel_tol, data_.operator_is_singular, s);
   else
@@ -1077,7 +1190,9 @@ void MultigridPreconditioner<Number>::coarse_solve(hipStream_t s)
   vec::dot(red_, 6, cr_.data(), cr_.data(), n, s);
   double const r0  = std::sqrt(red_.fetch(s, 7)[6]);
   double const tol = std::max(1e-12, data_.coarse_rel_tol * r0);
-  last_coarse_its_ = level_pcg<Number>(*C.op, coarse_inv_diag_.data(), cr_.data(), C.sol, tol,
+  JacobiData<Number> P;
+  P.inv_diag       = coarse_inv_diag_.data();
+  last_coarse_its_ = level_pcg<Number>(*C.op, P, cr_.data(), C.sol, tol,
                                        data_.coarse_max_iter, data_.operator_is_singular, red_, s,
                                        coarse_ws_);
 }
@@ -1197,6 +1312,8 @@ std::unique_ptr<PreconditionerBase> create_multigrid(std::shared_ptr<StructuredG
 
 template class LevelOperator<double>;
 template class LevelOperator<float>;
+template struct JacobiData<double>;
+template struct JacobiData<float>;
 template class Q1Operator<double>;
 template class Q1Operator<float>;
 template class CGOperator<double>;

@@ -14,6 +14,7 @@
 #include <memory>
 #include <vector>
 
+#include "block_kernels.h"
 #include "cg_kernels.h"
 #include "common.h"
 #include "grid.h"
@@ -37,6 +38,22 @@ public:
   }
   // true if vmult may allocate device memory or HIP objects (graph capture on first use)
   virtual bool vmult_allocates() const { return false; }
+};
+
+// BlockJacobiPreconditioner (block_jacobi_preconditioner.h) on the FP64 operator: the exact
+// inverses of the cells' diagonal blocks (matrix-based variant, operator_base.cpp:789-815); on DG
+// spaces it is also the AdditiveSchwarzPreconditioner (operator_base.cpp:2002-2009, 2099-2103)
+class BlockJacobiPreconditioner : public PreconditionerBase
+{
+public:
+  explicit BlockJacobiPreconditioner(Poisson::LaplaceOperator<double> const & op);
+  void vmult(double * dst, double const * src, hipStream_t s) override;
+  void update(hipStream_t s) override;
+
+private:
+  Poisson::LaplaceOperator<double> const & op_;
+  int                                      m_ = 0;
+  DeviceVector<double>                     blocks_;
 };
 
 // point Jacobi (jacobi_preconditioner.h:33-82)
@@ -80,6 +97,26 @@ struct MGLevelInfo
 std::vector<MGLevelInfo> multigrid_levels(MultigridType type, std::vector<int> const & degrees,
                                           int n_h_levels, bool coarse_q1);
 
+// PreconditionerSmoother (multigrid_parameters.h:161-167): the inner preconditioner of the
+// Chebyshev smoother
+enum class SmootherPreconditioner : int
+{
+  point_jacobi     = 0,
+  block_jacobi     = 1,  // DG levels only, as in the reference
+  additive_schwarz = 2   // on DG levels the block-Jacobi inverse (operator_base.cpp:2002-2009)
+};
+
+// point Jacobi (inv_diag) or block Jacobi (blocks: m x m inverse per cell) as a level
+// preconditioner z = P r
+template<typename Number>
+struct JacobiData
+{
+  Number const * inv_diag = nullptr;
+  Number const * blocks   = nullptr;
+  int            m        = 0;
+  void           apply(Number * z, Number const * r, int64_t n, hipStream_t s) const;
+};
+
 struct MultigridData
 {
   std::vector<int> degrees = {3, 1};     // p-sequence, fine to coarse (degrees[0] = k)
@@ -89,6 +126,7 @@ struct MultigridData
   // every cells-per-direction count is even and the result keeps at least 2
   int              n_h_levels = 0;
   int              smoother_degree = 5;  // SmootherData::iterations
+  SmootherPreconditioner smoother_preconditioner = SmootherPreconditioner::point_jacobi;
   double           smoothing_range = 20; // SmootherData::smoothing_range
   int              eig_cg_iterations = 20;
   CoarseSolver     coarse_solver   = CoarseSolver::cg;
@@ -125,6 +163,12 @@ public:
   virtual bool    continuous() const { return false; }
   // node numbering of a continuous level (null for DG levels)
   virtual CGTopology const * cg_topology() const { return nullptr; }
+  // cell block size n^3 of a DG level (0: no block diagonal) and the inverse cell blocks
+  virtual int  block_size() const { return 0; }
+  virtual void inverse_block_diagonal(Number * /*blocks*/, hipStream_t /*s*/) const
+  {
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "Block Jacobi only implemented for DG!");
+  }
   // dst = b - A src; operators with a fused epilogue override this (the default applies A and
   // then updates, so it needs dst != b)
   virtual void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const;
@@ -206,7 +250,8 @@ class ChebyshevSmoother
 {
 public:
   void setup(LevelOperator<Number> const & op, int degree, double smoothing_range,
-             int eig_cg_iterations, hipStream_t s);
+             int eig_cg_iterations, hipStream_t s,
+             SmootherPreconditioner prec = SmootherPreconditioner::point_jacobi);
   // coarse-grid variant: interval [lambda_min, 1.1 lambda_max] from a CG estimate to 1e-5
   void setup_coarse(LevelOperator<Number> const & op, double rel_tol, bool singular, hipStream_t s);
   // x: current solution buffer, x_alt: second buffer; both may be swapped on return
@@ -221,8 +266,12 @@ private:
   void apply_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
                   Number f1, Number f2, bool has_old, hipStream_t s) const;
 
+  JacobiData<Number> jacobi_data() const;
+
   LevelOperator<Number> const * op_ = nullptr;
   DeviceVector<Number>          inv_diag_;
+  DeviceVector<Number>          blocks_;  // block Jacobi: inverse cell blocks
+  int                           block_m_ = 0;
   mutable DeviceVector<Number>  t_;
 };
 

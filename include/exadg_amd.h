@@ -139,9 +139,22 @@ int exadg_laplace_rhs(exadg_laplace * op, const double * f, const double * bound
 
 enum exadg_preconditioner_kind
 {
-  EXADG_PRECOND_NONE         = 0,
-  EXADG_PRECOND_POINT_JACOBI = 1,
-  EXADG_PRECOND_MULTIGRID    = 2
+  EXADG_PRECOND_NONE             = 0,
+  EXADG_PRECOND_POINT_JACOBI     = 1,
+  EXADG_PRECOND_MULTIGRID        = 2,
+  EXADG_PRECOND_BLOCK_JACOBI     = 3, /* exact inverse cell blocks, degree <= 4             */
+  EXADG_PRECOND_ADDITIVE_SCHWARZ = 4  /* on the DG space: the block-Jacobi inverse          */
+};
+
+/* PreconditionerSmoother (multigrid_parameters.h:161-167): the inner preconditioner of the
+ * Chebyshev smoother. BLOCK_JACOBI and ADDITIVE_SCHWARZ need DG levels of degree <= 4 (the
+ * reference refuses block Jacobi on continuous levels; additive Schwarz on continuous levels is
+ * not implemented here). */
+enum exadg_smoother_preconditioner
+{
+  EXADG_SMOOTHER_POINT_JACOBI     = 0,
+  EXADG_SMOOTHER_BLOCK_JACOBI     = 1,
+  EXADG_SMOOTHER_ADDITIVE_SCHWARZ = 2
 };
 
 enum exadg_coarse_solver
@@ -198,6 +211,7 @@ typedef struct exadg_mg_desc
                                 MI355X the replay measured no faster than eager launches */
   int32_t mg_type;           /* exadg_mg_type (default EXADG_MG_LEGACY)                */
   int32_t n_h_levels;        /* meshes of the h-levels, 0 = automatic (default)        */
+  int32_t smoother_preconditioner; /* exadg_smoother_preconditioner (default point Jacobi) */
 } exadg_mg_desc;
 
 /* Defaults of the reference for degree k: PSequenceType::Bisect (p -> max(p/2,1),
@@ -208,12 +222,16 @@ int exadg_mg_desc_default(int degree, exadg_mg_desc * out);
 typedef struct exadg_preconditioner exadg_preconditioner;
 
 /* The preconditioner factory of Poisson::Operator::setup_preconditioner_and_solver
- * (operator.cpp:306-398) for None / PointJacobi / Multigrid; `mg` is read for MULTIGRID only.
+ * (operator.cpp:306-398) for None / PointJacobi / BlockJacobi / AdditiveSchwarz / Multigrid;
+ * `mg` is read for MULTIGRID only. BlockJacobi (block_jacobi_preconditioner.h) applies the exact
+ * inverses of the cells' diagonal blocks A_KK, computed through the cell-based loop and inverted
+ * once at setup (matrix-based variant, operator_base.cpp:789-815); degree <= 4.
  * The operator must be FP64 (the outer solver's precision). Setup includes the diagonals,
  * the Chebyshev eigenvalue estimates and the coarse solver (PreconditionerBase::update).
- * Lifetime: a POINT_JACOBI preconditioner refers to `op` (as the reference's
- * JacobiPreconditioner holds a reference to its operator, jacobi_preconditioner.h:33-82), so
- * `op` must outlive it; a MULTIGRID one shares the operator's grid and owns its levels. */
+ * Lifetime: a POINT_JACOBI, BLOCK_JACOBI or ADDITIVE_SCHWARZ preconditioner refers to `op` (as
+ * the reference's JacobiPreconditioner and BlockJacobiPreconditioner hold a reference to their
+ * operator, jacobi_preconditioner.h:33-82), so `op` must outlive it; a MULTIGRID one shares the
+ * operator's grid and owns its levels. */
 int exadg_preconditioner_create(exadg_laplace * op, int kind, const exadg_mg_desc * mg,
                                 exadg_preconditioner ** out, void * stream);
 /* PreconditionerBase::vmult (preconditioner_base.h:33-67): dst = P^{-1} src, FP64 vectors. */

@@ -110,7 +110,12 @@ def lib():
         L.oracle_mg_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(ctypes.c_int),
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int]
+        L.oracle_laplace_block_diagonal.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_cg_solve_block_jacobi.restype = ctypes.c_int
+        L.oracle_cg_solve_block_jacobi.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int, dp]
         L.oracle_mg_level_list.restype = ctypes.c_int
         L.oracle_mg_level_list.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int] + \
@@ -209,6 +214,24 @@ class LaplaceOperator:
         lib().oracle_laplace_diagonal(self._h, _p(d))
         return d
 
+    def block_diagonal(self):
+        """the cells' diagonal blocks A_KK, shape (n_cells, n^3, n^3), [c, j, i] = (A_KK)_ij"""
+        m = (self.degree + 1) ** 3
+        blocks = np.empty((self.n // m, m, m))
+        lib().oracle_laplace_block_diagonal(self._h, _p(blocks))
+        return blocks
+
+    def cg_solve_block_jacobi(self, b, x0=None, rel_tol=1e-10, abs_tol=1e-20, max_iter=100000):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        info = np.zeros(4)
+        rc = lib().oracle_cg_solve_block_jacobi(self._h, _p(b), _p(x), rel_tol, abs_tol, max_iter,
+                                                _p(info))
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        return x, dict(iterations=int(info[0]), r0=info[1], r_final=info[2],
+                       converged=bool(info[3]))
+
     def tau(self):
         nc = self.n // (self.degree + 1) ** 3
         t = np.empty(nc)
@@ -271,13 +294,14 @@ class Multigrid:
 
     def __init__(self, mesh, degrees, coarse_q1=True, smoother_degree=5, smoothing_range=20.0,
                  eig_its=20, coarse_solver=0, coarse_rel_tol=1e-3, coarse_max_iter=10000,
-                 singular=False, ip_factor=1.0, mg_type=0, n_h_levels=1):
+                 singular=False, ip_factor=1.0, mg_type=0, n_h_levels=1,
+                 smoother_preconditioner=0):
         t = MG_TYPES.index(mg_type) if isinstance(mg_type, str) else int(mg_type)
         arr = (ctypes.c_int * len(degrees))(*degrees)
         self._h = lib().oracle_mg_create(ctypes.byref(mesh), arr, len(degrees), int(coarse_q1),
                                          smoother_degree, smoothing_range, eig_its, coarse_solver,
                                          coarse_rel_tol, coarse_max_iter, int(singular), ip_factor,
-                                         t, n_h_levels)
+                                         t, n_h_levels, int(smoother_preconditioner))
         if not self._h:
             raise RuntimeError(lib().oracle_last_error().decode())
         self.info()

@@ -146,11 +146,44 @@ int oracle_cg_solve(void * h, int use_jacobi, double const * b, double * x, doub
   }
 }
 
-// hybrid multigrid (p-sequence: degrees fine to coarse; type: MGType; n_h_levels meshes)
+// the cells' diagonal blocks (n_cells x n^3 x n^3, column j of cell c at (c n^3 + j) n^3)
+void oracle_laplace_block_diagonal(void * h, double * blocks)
+{
+  static_cast<LaplaceOperator *>(h)->block_diagonal(blocks);
+}
+
+// PCG preconditioned by block Jacobi (Cholesky-factorized cell blocks)
+int oracle_cg_solve_block_jacobi(void * h, double const * b, double * x, double rel_tol,
+                                 double abs_tol, int max_iter, double * info)
+{
+  auto * op = static_cast<LaplaceOperator *>(h);
+  try
+  {
+    BlockJacobi   bj(*op);
+    int64_t const n = op->n_dofs();
+    Apply         A = [&](double * dst, double const * src) { op->vmult(dst, src); };
+    Apply         P = [&](double * dst, double const * src) { bj.vmult(dst, src); };
+    SolveInfo     si = pcg(n, A, &P, b, x, rel_tol, abs_tol, max_iter);
+    info[0]          = si.iterations;
+    info[1]          = si.r0;
+    info[2]          = si.r_final;
+    info[3]          = si.converged ? 1.0 : 0.0;
+    return 0;
+  }
+  catch(std::exception const & e)
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// hybrid multigrid (p-sequence: degrees fine to coarse; type: MGType; n_h_levels meshes;
+// smoother preconditioner 0 point Jacobi, 1 block Jacobi, 2 additive Schwarz)
 void * oracle_mg_create(oracle_mesh const * m, int const * degrees, int n_degrees, int coarse_q1,
                         int smoother_degree, double smoothing_range, int eig_its,
                         int coarse_solver, double coarse_rel_tol, int coarse_max_iter,
-                        int singular, double ip_factor, int type, int n_h_levels)
+                        int singular, double ip_factor, int type, int n_h_levels,
+                        int smoother_preconditioner)
 {
   try
   {
@@ -161,6 +194,7 @@ void * oracle_mg_create(oracle_mesh const * m, int const * degrees, int n_degree
     md.coarse_q1       = coarse_q1 != 0;
     md.type            = type;
     md.n_h_levels      = n_h_levels;
+    md.smoother_preconditioner = smoother_preconditioner;
     md.smoother_degree = smoother_degree;
     md.smoothing_range = smoothing_range;
     md.eig_its         = eig_its;

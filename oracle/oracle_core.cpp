@@ -914,10 +914,15 @@ double LaplaceOperator::error_sine(double omega, double const * u) const
   return std::sqrt(err2) / std::sqrt(nrm2);
 }
 
-void LaplaceOperator::diagonal(double * dg) const
+void LaplaceOperator::diagonal(double * dg) const { probe_blocks(dg, nullptr); }
+
+void LaplaceOperator::block_diagonal(double * blocks) const { probe_blocks(nullptr, blocks); }
+
+void LaplaceOperator::probe_blocks(double * dg, double * blocks) const
 {
-  // diag(A): cell term plus the self-coupling of every face of the cell (neighbour values
-  // zero), probed with unit vectors (MatrixFreeTools::compute_diagonal semantics)
+  // diag(A) or the whole diagonal block A_KK: cell term plus the self-coupling of every face of
+  // the cell (neighbour values zero), probed with unit vectors (MatrixFreeTools::compute_diagonal
+  // semantics; OperatorBase::add_block_diagonal_matrices, operator_base.cpp:649-680)
   int64_t const nc         = mesh.n_cells;
   int const     n2         = n * n;
   double const  tau_factor = penalty_factor();
@@ -1010,7 +1015,10 @@ void LaplaceOperator::diagonal(double * dg) const
           }
           face_integrate(dir, side, rv.data(), rg.data(), out.data());
         }
-        dg[c * n3 + i] = out[i];
+        if(dg)
+          dg[c * n3 + i] = out[i];
+        if(blocks)  // column i of the cell's block
+          std::copy(out.begin(), out.end(), blocks + ((size_t)c * n3 + i) * n3);
       }
     }
   }

@@ -131,6 +131,11 @@ struct LaplaceOperator
   double error_sine(double omega, double const * u) const;
   // exact diagonal by unit-vector probing of the local cell/face matrices
   void diagonal(double * d) const;
+  // the n^3 x n^3 diagonal block A_KK of every cell (cell term plus the self-coupling of its
+  // faces, neighbours zero), column j of cell c at blocks[(c n^3 + j) n^3 ...]
+  void block_diagonal(double * blocks) const;
+  // both by one probing loop (either pointer may be null)
+  void probe_blocks(double * dg, double * blocks) const;
 };
 
 // b = weak DG divergence of the discrete velocity u (FE_DGQ(degree_u)^3: per cell three

@@ -350,6 +350,61 @@ void CGOperator::diagonal(double * dg) const
       dg[g] = 1.0;
 }
 
+// ---------------------------------------------------------------- block Jacobi
+
+BlockJacobi::BlockJacobi(LaplaceOperator const & op) : m(op.n3), n_cells(op.mesh.n_cells)
+{
+  L.resize((size_t)n_cells * m * m);
+  op.block_diagonal(L.data());
+#pragma omp parallel for schedule(static)
+  for(int64_t c = 0; c < n_cells; ++c)
+  {
+    double * A = &L[(size_t)c * m * m];  // symmetric: column storage = row storage
+    for(int j = 0; j < m; ++j)
+    {
+      double d = A[j * m + j];
+      for(int k = 0; k < j; ++k)
+        d -= A[j * m + k] * A[j * m + k];
+      if(!(d > 0))
+        throw std::runtime_error("block Jacobi: cell block not positive definite");
+      double const ljj = std::sqrt(d);
+      A[j * m + j]     = ljj;
+      for(int i = j + 1; i < m; ++i)
+      {
+        double v = A[i * m + j];
+        for(int k = 0; k < j; ++k)
+          v -= A[i * m + k] * A[j * m + k];
+        A[i * m + j] = v / ljj;
+      }
+    }
+  }
+}
+
+void BlockJacobi::vmult(double * dst, double const * src) const
+{
+#pragma omp parallel for schedule(static)
+  for(int64_t c = 0; c < n_cells; ++c)
+  {
+    double const * A = &L[(size_t)c * m * m];
+    double *       y = dst + c * m;
+    double const * r = src + c * m;
+    for(int i = 0; i < m; ++i)  // L y = r
+    {
+      double v = r[i];
+      for(int k = 0; k < i; ++k)
+        v -= A[i * m + k] * y[k];
+      y[i] = v / A[i * m + i];
+    }
+    for(int i = m - 1; i >= 0; --i)  // L^T x = y
+    {
+      double v = y[i];
+      for(int k = i + 1; k < m; ++k)
+        v -= A[k * m + i] * y[k];
+      y[i] = v / A[i * m + i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- Chebyshev
 
 static Apply jacobi(std::vector<double> const & inv_diag)
@@ -363,13 +418,15 @@ static Apply jacobi(std::vector<double> const & inv_diag)
 
 void Chebyshev::setup_smoother(Apply const & A_, std::vector<double> dinv,
                                std::vector<char> const & mask, int degree_, double range,
-                               int eig_its)
+                               int eig_its, Apply const & P_)
 {
   A        = A_;
   inv_diag = std::move(dinv);
+  P        = P_;
   degree   = degree_;
   int64_t const n = (int64_t)inv_diag.size();
-  // deal.II initial vector: (i mod 11) minus mean, constrained entries zero
+  // deal.II initial vector: (i mod 11) minus mean (entries in `mask` zero; the multigrid passes
+  // none, as ExaDG's smoother sets no constraints object)
   std::vector<double> b(n), x(n);
   for(int64_t i = 0; i < n; ++i)
     b[i] = (double)(i % 11);
@@ -379,8 +436,8 @@ void Chebyshev::setup_smoother(Apply const & A_, std::vector<double> dinv,
       b[i] = 0;
   std::vector<double> al, be;
   double const        nb = std::sqrt(dotv(n, b.data(), b.data()));
-  cg_iterations = pcg_record(n, A, jacobi(inv_diag), b.data(), x.data(), 1e-2 * nb, eig_its, false,
-                             &al, &be);
+  cg_iterations = pcg_record(n, A, P ? P : jacobi(inv_diag), b.data(), x.data(), 1e-2 * nb,
+                             eig_its, false, &al, &be);
   tridiagonal_extreme_eigenvalues(al, be, lambda_min, lambda_max);
   double const max_ev = 1.2 * lambda_max;
   double const lo     = max_ev / range;
@@ -416,25 +473,39 @@ void Chebyshev::setup_coarse(Apply const & A_, std::vector<double> dinv, double 
   delta           = 0.5 * (max_ev - lo);
 }
 
+// z = P^{-1} r with the smoother's inner preconditioner
+static void precondition(Chebyshev const & c, std::vector<double> & z, std::vector<double> const & r)
+{
+  if(c.P)
+    c.P(z.data(), r.data());
+  else
+    for(size_t i = 0; i < r.size(); ++i)
+      z[i] = c.inv_diag[i] * r[i];
+}
+
 void Chebyshev::vmult(std::vector<double> & x, std::vector<double> const & b) const
 {
   // deal.II PreconditionChebyshev::vmult: zero initial guess
   int64_t const       n = (int64_t)b.size();
-  std::vector<double> xo(n, 0.0), t(n);
+  std::vector<double> xo(n, 0.0), t(n), r(n), z(n);
+  precondition(*this, z, b);
   for(int64_t i = 0; i < n; ++i)
-    x[i] = inv_diag[i] * b[i] / theta;
+    x[i] = z[i] / theta;
   if(degree < 2 || std::fabs(delta) < 1e-40)
     return;
   double rhok = delta / theta, sigma = theta / delta;
   for(int it = 1; it < degree; ++it)
   {
     A(t.data(), x.data());
+    for(int64_t i = 0; i < n; ++i)
+      r[i] = b[i] - t[i];
+    precondition(*this, z, r);
     double const rhokp = 1. / (2. * sigma - rhok);
     double const f1 = rhokp * rhok, f2 = 2. * rhokp / delta;
     rhok            = rhokp;
     for(int64_t i = 0; i < n; ++i)
     {
-      double const xn = (1. + f1) * x[i] - (it > 1 ? f1 * xo[i] : 0.0) + f2 * inv_diag[i] * (b[i] - t[i]);
+      double const xn = (1. + f1) * x[i] - (it > 1 ? f1 * xo[i] : 0.0) + f2 * z[i];
       xo[i]           = x[i];
       x[i]            = xn;
     }
@@ -444,12 +515,15 @@ void Chebyshev::vmult(std::vector<double> & x, std::vector<double> const & b) co
 void Chebyshev::step(std::vector<double> & x, std::vector<double> const & b) const
 {
   int64_t const       n = (int64_t)b.size();
-  std::vector<double> xo(n), t(n);
+  std::vector<double> xo(n), t(n), r(n), z(n);
   A(t.data(), x.data());
+  for(int64_t i = 0; i < n; ++i)
+    r[i] = b[i] - t[i];
+  precondition(*this, z, r);
   for(int64_t i = 0; i < n; ++i)
   {
     xo[i] = x[i];
-    x[i]  = x[i] + inv_diag[i] * (b[i] - t[i]) / theta;
+    x[i]  = x[i] + z[i] / theta;
   }
   if(degree < 2 || std::fabs(delta) < 1e-40)
     return;
@@ -457,12 +531,15 @@ void Chebyshev::step(std::vector<double> & x, std::vector<double> const & b) con
   for(int it = 1; it < degree; ++it)
   {
     A(t.data(), x.data());
+    for(int64_t i = 0; i < n; ++i)
+      r[i] = b[i] - t[i];
+    precondition(*this, z, r);
     double const rhokp = 1. / (2. * sigma - rhok);
     double const f1 = rhokp * rhok, f2 = 2. * rhokp / delta;
     rhok            = rhokp;
     for(int64_t i = 0; i < n; ++i)
     {
-      double const xn = (1. + f1) * x[i] - f1 * xo[i] + f2 * inv_diag[i] * (b[i] - t[i]);
+      double const xn = (1. + f1) * x[i] - f1 * xo[i] + f2 * z[i];
       xo[i]           = x[i];
       x[i]            = xn;
     }
@@ -640,8 +717,17 @@ Multigrid::Multigrid(MeshDesc const & m, MGDesc const & d, double ip) : mesh(m),
     if(k == 0)
       continue;
     std::vector<char> mask;
+    Apply             P;
+    if(d.smoother_preconditioner != 0)
+    {
+      if(!L.dg)
+        throw std::runtime_error("block Jacobi / additive Schwarz smoothing needs DG levels");
+      L.block           = std::make_unique<BlockJacobi>(*L.dg_op);
+      BlockJacobi * blk = L.block.get();
+      P                 = [blk](double * dst, double const * src) { blk->vmult(dst, src); };
+    }
     L.smoother.setup_smoother(L.A, L.inv_diag, mask, d.smoother_degree, d.smoothing_range,
-                              d.eig_its);
+                              d.eig_its, P);
     Level const & C = levels[k - 1];
     if(C.dg != L.dg)
       L.transfer = 'c';

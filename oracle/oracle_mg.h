@@ -66,16 +66,29 @@ struct CGOperator
   void    diagonal(double * d) const;
 };
 
+// BlockJacobiPreconditioner (block_jacobi_preconditioner.h), matrix-based: the cells' diagonal
+// blocks A_KK factorized by Cholesky once, applied by forward and back substitution
+struct BlockJacobi
+{
+  int                 m       = 0;
+  int64_t             n_cells = 0;
+  std::vector<double> L;  // per cell the lower Cholesky factor, row-major m x m
+  explicit BlockJacobi(LaplaceOperator const & op);
+  void vmult(double * dst, double const * src) const;
+};
+
 struct Chebyshev
 {
   Apply               A;
   std::vector<double> inv_diag;
+  // inner preconditioner other than point Jacobi (block Jacobi); empty: inv_diag
+  Apply               P;
   int                 degree = 5;
   double              theta = 0, delta = 0, lambda_max = 0, lambda_min = 0;
   int                 cg_iterations = 0;
   // smoother setup (deal.II PreconditionChebyshev::estimate_eigenvalues)
   void setup_smoother(Apply const & A, std::vector<double> inv_diag, std::vector<char> const & mask,
-                      int degree, double range, int eig_its);
+                      int degree, double range, int eig_its, Apply const & P = Apply());
   // MGCoarseChebyshev::update
   void setup_coarse(Apply const & A, std::vector<double> inv_diag, double rel_tol, bool singular);
   void vmult(std::vector<double> & x, std::vector<double> const & b) const;
@@ -115,6 +128,8 @@ struct MGDesc
   double           coarse_rel_tol = 1e-3;
   int              coarse_max_iter = 10000;
   bool             singular = false;
+  int              smoother_preconditioner = 0;  // 0 point Jacobi, 1 block Jacobi, 2 additive
+                                                 // Schwarz (= block Jacobi on DG levels)
 };
 
 struct Multigrid
@@ -130,6 +145,7 @@ struct Multigrid
     std::unique_ptr<LaplaceOperator>        dg_op;
     std::unique_ptr<Q1Operator>             q1_op;
     std::unique_ptr<CGOperator>             cg_op;
+    std::unique_ptr<BlockJacobi>            block;
     Apply                                   A;
     std::vector<double>                     inv_diag;
     Chebyshev                               smoother;
