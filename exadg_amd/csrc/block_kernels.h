@@ -24,12 +24,20 @@ constexpr int kMaxDenseBlock = 125;
 template<typename Number>
 hipError_t launch_block_invert(Number * blocks, int m, int64_t n_cells, hipStream_t s);
 
-// dst = B r per cell (B symmetric, m x m per cell), where r = b - t (t may be NULL: r = b);
+// dst = B r per cell, where r = b - t (t may be NULL: r = b) and B the inverse cell block;
 // variant 0: dst = f2 B r; 1: dst = (1+f1) x + f2 B r; 2: dst = (1+f1) x - f1 x_old + f2 B r.
+// B is either dense (blocks: m x m per cell, symmetric) or, on affine cells, given by fast
+// diagonalization: B = (V_z x V_y x V_x) diag(1 / (l_x + l_y + l_z)) (V_z x V_y x V_x)^T with
+// the 1D generalized eigenvectors V (n x n, row = node) and eigenvalues l of the cell's three
+// 1D configurations (fdm_cfg: 3 indices per cell into fdm_vec / fdm_val).
 template<typename Number>
 struct BlockApplyArgs
 {
-  Number const * blocks = nullptr;
+  Number const * blocks  = nullptr;
+  Number const * fdm_vec = nullptr;
+  Number const * fdm_val = nullptr;
+  int const *    fdm_cfg = nullptr;
+  int            n1d     = 0;
   Number const * b      = nullptr;
   Number const * t      = nullptr;
   Number const * x      = nullptr;

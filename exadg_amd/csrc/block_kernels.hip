@@ -4,6 +4,7 @@
 #include <atomic>
 
 #include "block_kernels.h"
+#include "line_sweeps.h"
 
 namespace ExaDG
 {
@@ -86,6 +87,93 @@ __global__ void __launch_bounds__(256) block_apply_kernel(BlockApplyArgs<Number>
     a.dst[g] = v;
   }
 }
+// Fast-diagonalization inverse of the affine cell blocks: one group of n^2 lanes per cell,
+// six 1D sweeps in LDS (V^T along x, y, z, the division by the eigenvalue sums, V along z, y, x).
+template<typename Number, int N>
+__global__ void __launch_bounds__(256) fdm_apply_kernel(BlockApplyArgs<Number> a, int cpb)
+{
+  using namespace sweeps;
+  constexpr int N2 = N * N, N3 = N2 * N;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Number *      sm     = reinterpret_cast<Number *>(smem_raw);
+  int const     slot   = threadIdx.x / N2;
+  int const     lt     = threadIdx.x - slot * N2;
+  int64_t const cell   = (int64_t)blockIdx.x * cpb + slot;
+  bool const    active = slot < cpb && cell < a.n_cells;
+  Number *      U      = sm + (active ? slot : 0) * 2 * N3;
+  Number *      T      = U + N3;
+  int           cfg[3] = {0, 0, 0};
+  if(active)
+  {
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+      cfg[d] = a.fdm_cfg[cell * 3 + d];
+    for(int e = lt; e < N3; e += N2)
+    {
+      int64_t const g = cell * N3 + e;
+      U[e]            = a.t ? a.b[g] - a.t[g] : a.b[g];
+    }
+  }
+  __syncthreads();
+  if(active)
+    sweep<Number, N, true, false>(a.fdm_vec + cfg[0] * N2, U, T, 0, lt);
+  __syncthreads();
+  if(active)
+    sweep<Number, N, true, false>(a.fdm_vec + cfg[1] * N2, T, U, 1, lt);
+  __syncthreads();
+  if(active)
+  {
+    // z-line of this lane: transform, divide by the eigenvalue sums, transform back
+    int const      base = line_base<N>(2, lt);
+    Number const * vz   = a.fdm_vec + cfg[2] * N2;
+    Number const   lx   = a.fdm_val[cfg[0] * N + lt % N];
+    Number const   ly   = a.fdm_val[cfg[1] * N + lt / N];
+    Number         x[N], y[N];
+#pragma unroll
+    for(int i = 0; i < N; ++i)
+      x[i] = U[base + i * N2];
+    mat_line<Number, N, true>(vz, x, y);
+#pragma unroll
+    for(int c = 0; c < N; ++c)
+      y[c] /= lx + ly + a.fdm_val[cfg[2] * N + c];
+    mat_line<Number, N, false>(vz, y, x);
+#pragma unroll
+    for(int i = 0; i < N; ++i)
+      U[base + i * N2] = x[i];
+  }
+  __syncthreads();
+  if(active)
+    sweep<Number, N, false, false>(a.fdm_vec + cfg[1] * N2, U, T, 1, lt);
+  __syncthreads();
+  if(active)
+    sweep<Number, N, false, false>(a.fdm_vec + cfg[0] * N2, T, U, 0, lt);
+  __syncthreads();
+  if(active)
+    for(int e = lt; e < N3; e += N2)
+    {
+      int64_t const g = cell * N3 + e;
+      Number const  z = U[e];
+      if(a.variant == 0)
+        a.dst[g] = a.f2 * z;
+      else
+      {
+        Number v = (Number(1) + a.f1) * a.x[g] + a.f2 * z;
+        if(a.variant == 2)
+          v -= a.f1 * a.x_old[g];
+        a.dst[g] = v;
+      }
+    }
+}
+
+template<typename Number, int N>
+hipError_t launch_fdm_n(BlockApplyArgs<Number> const & a, hipStream_t s)
+{
+  int const     cpb = 256 / (N * N);
+  size_t const  lds = sizeof(Number) * cpb * 2 * N * N * N;
+  int64_t const nb  = (a.n_cells + cpb - 1) / cpb;
+  hipLaunchKernelGGL((fdm_apply_kernel<Number, N>), dim3((unsigned)nb), dim3(256), lds, s, a, cpb);
+  return hipGetLastError();
+}
 }  // namespace
 
 template<typename Number>
@@ -114,6 +202,19 @@ hipError_t launch_block_invert(Number * blocks, int m, int64_t n_cells, hipStrea
 template<typename Number>
 hipError_t launch_block_apply(BlockApplyArgs<Number> const & a, hipStream_t s)
 {
+  if(a.fdm_vec)
+    switch(a.n1d)
+    {
+      case 2: return launch_fdm_n<Number, 2>(a, s);
+      case 3: return launch_fdm_n<Number, 3>(a, s);
+      case 4: return launch_fdm_n<Number, 4>(a, s);
+      case 5: return launch_fdm_n<Number, 5>(a, s);
+      case 6: return launch_fdm_n<Number, 6>(a, s);
+      case 7: return launch_fdm_n<Number, 7>(a, s);
+      case 8: return launch_fdm_n<Number, 8>(a, s);
+      case 9: return launch_fdm_n<Number, 9>(a, s);
+      default: return hipErrorInvalidValue;
+    }
   if(a.m < 1 || a.m > 256)
     return hipErrorInvalidValue;
   int const     lanes = (a.m + 63) / 64 * 64;

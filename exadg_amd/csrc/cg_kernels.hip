@@ -5,6 +5,7 @@
 
 #include "../../include/exadg_amd.h"
 #include "cg_kernels.h"
+#include "line_sweeps.h"
 
 namespace ExaDG
 {
@@ -27,54 +28,10 @@ namespace
 {
 inline int blocks_for(int64_t n) { return (int)((n + 255) / 256); }
 
-// ---------------------------------------------------------------- 1D line sweeps in LDS
-//
-// An n^3 tensor (x fastest) is processed as n^2 lines along direction d; line `line` enumerates
-// the two other coordinates (lower one fastest).
-template<int N>
-__device__ __forceinline__ int line_base(int d, int line)
-{
-  int const e0 = line % N, e1 = line / N;
-  return d == 0 ? N * (e0 + N * e1) : (d == 1 ? e0 + N * N * e1 : e0 + N * e1);
-}
-
-template<int N>
-__device__ __forceinline__ constexpr int line_stride(int d)
-{
-  return d == 0 ? 1 : (d == 1 ? N : N * N);
-}
-
-// y[a] = sum_i M(a, i) x[i] with M(a, i) = M[a N + i], or M[i N + a] when T
-template<typename Number, int N, bool T>
-__device__ __forceinline__ void mat_line(Number const * __restrict__ M, Number const (&x)[N],
-                                         Number (&y)[N])
-{
-#pragma unroll
-  for(int a = 0; a < N; ++a)
-  {
-    Number s = 0;
-#pragma unroll
-    for(int i = 0; i < N; ++i)
-      s += (T ? M[i * N + a] : M[a * N + i]) * x[i];
-    y[a] = s;
-  }
-}
-
-// out (line of this lane) = M applied along d to in; ACC adds to out
-template<typename Number, int N, bool T, bool ACC>
-__device__ __forceinline__ void sweep(Number const * __restrict__ M, Number const * in, Number * out,
-                                      int d, int line)
-{
-  int const base = line_base<N>(d, line), st = line_stride<N>(d);
-  Number    x[N], y[N];
-#pragma unroll
-  for(int i = 0; i < N; ++i)
-    x[i] = in[base + i * st];
-  mat_line<Number, N, T>(M, x, y);
-#pragma unroll
-  for(int a = 0; a < N; ++a)
-    out[base + a * st] = ACC ? out[base + a * st] + y[a] : y[a];
-}
+using sweeps::line_base;
+using sweeps::line_stride;
+using sweeps::mat_line;
+using sweeps::sweep;
 
 // ---------------------------------------------------------------- c-transfer (any degree)
 

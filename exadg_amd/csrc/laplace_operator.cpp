@@ -3,6 +3,8 @@
 #include "block_kernels.h"
 
 #include <algorithm>
+#include <tuple>
+#include <map>
 #include <cmath>
 #include <type_traits>
 
@@ -247,6 +249,265 @@ void LaplaceOperator<Number>::calculate_inverse_diagonal(Number * diag, hipStrea
 {
   calculate_diagonal(diag, s);
   invert_diagonal_device<Number>(diag, m(), s);
+}
+
+namespace
+{
+// eigenvalues and eigenvectors of a symmetric n x n matrix (cyclic Jacobi rotations); on return
+// A holds the eigenvalues on its diagonal and Q (row-major) the eigenvectors as columns
+void symmetric_eigen(std::vector<double> & A, std::vector<double> & Q, int n)
+{
+  Q.assign((size_t)n * n, 0.0);
+  for(int i = 0; i < n; ++i)
+    Q[i * n + i] = 1.0;
+  for(int sweep = 0; sweep < 100; ++sweep)
+  {
+    double off = 0, diag = 0;
+    for(int i = 0; i < n; ++i)
+      for(int j = 0; j < n; ++j)
+        (i == j ? diag : off) += A[i * n + j] * A[i * n + j];
+    if(off <= 1e-30 * diag)
+      return;
+    for(int p = 0; p < n; ++p)
+      for(int q = p + 1; q < n; ++q)
+      {
+        double const apq = A[p * n + q];
+        if(apq == 0.0)
+          continue;
+        double const theta = 0.5 * (A[q * n + q] - A[p * n + p]) / apq;
+        double const t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double const c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for(int k = 0; k < n; ++k)  // A <- A R (columns p, q)
+        {
+          double const akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p]     = c * akp - sn * akq;
+          A[k * n + q]     = sn * akp + c * akq;
+        }
+        for(int k = 0; k < n; ++k)  // A <- R^T A (rows p, q)
+        {
+          double const apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k]     = c * apk - sn * aqk;
+          A[q * n + k]     = sn * apk + c * aqk;
+        }
+        for(int k = 0; k < n; ++k)
+        {
+          double const qkp = Q[k * n + p], qkq = Q[k * n + q];
+          Q[k * n + p]     = c * qkp - sn * qkq;
+          Q[k * n + q]     = sn * qkp + c * qkq;
+        }
+      }
+  }
+  EXADG_THROW(EXADG_ERR_INTERNAL, "Jacobi eigenvalue iteration did not converge");
+}
+
+// the 1D cell-block operator along one direction (the line Laplacian of the affine kernel with
+// zero neighbour values, or the cell's own values across a periodic face of a one-cell
+// direction), and its generalized eigen-decomposition with the mass h M
+void cell_line_eigen(ShapeInfo1D const & sh, double h, int type_l, int type_r, bool self, double tau_l,
+                     double tau_r, double * vec, double * val)
+{
+  int const           n = sh.n;
+  std::vector<double> L((size_t)n * n), u(n), o(n);
+  for(int j = 0; j < n; ++j)
+  {
+    std::fill(u.begin(), u.end(), 0.0);
+    u[j] = 1.0;
+    for(int i = 0; i < n; ++i)
+    {
+      o[i] = 0;
+      for(int k = 0; k < n; ++k)
+        o[i] += sh.stiff[i * n + k] * u[k];
+    }
+    double du0 = 0, du1 = 0;
+    for(int i = 0; i < n; ++i)
+    {
+      du0 += sh.d0[i] * u[i];
+      du1 += sh.d1[i] * u[i];
+    }
+    auto exterior = [&](int type, bool left, double & w, double & dw) {
+      double const ue = left ? u[0] : u[n - 1], de = left ? du0 : du1;
+      if(type == EXADG_BC_DIRICHLET)
+        w = -ue, dw = de;
+      else if(type == EXADG_BC_NEUMANN)
+        w = ue, dw = -de;
+      else if(self)
+        w = left ? u[n - 1] : u[0], dw = left ? du1 : du0;
+      else
+        w = 0, dw = 0;
+    };
+    double wl, dwl, wr, dwr;
+    exterior(type_l, true, wl, dwl);
+    exterior(type_r, false, wr, dwr);
+    double const jl = 0.5 * (u[0] - wl), jr = 0.5 * (u[n - 1] - wr);
+    for(int i = 0; i < n; ++i)
+      o[i] = (o[i] + sh.d0[i] * jl - sh.d1[i] * jr) / h;
+    o[0] += 0.5 * (du0 + dwl) / h + 2.0 * tau_l * jl;
+    o[n - 1] += 2.0 * tau_r * jr - 0.5 * (du1 + dwr) / h;
+    for(int i = 0; i < n; ++i)
+      L[i * n + j] = o[i];
+  }
+  double asym = 0, scale = 0;
+  for(int i = 0; i < n; ++i)
+    for(int j = 0; j < n; ++j)
+    {
+      asym  = std::max(asym, std::fabs(L[i * n + j] - L[j * n + i]));
+      scale = std::max(scale, std::fabs(L[i * n + j]));
+    }
+  if(asym > 1e-10 * scale)
+    EXADG_THROW(EXADG_ERR_INTERNAL, "1D cell block is not symmetric");
+  // h M = C C^T (Cholesky), B = C^{-1} L C^{-T}, B = Q diag Q^T, V = C^{-T} Q
+  std::vector<double> C((size_t)n * n, 0.0);
+  for(int j = 0; j < n; ++j)
+  {
+    double d = h * sh.mass[j * n + j];
+    for(int k = 0; k < j; ++k)
+      d -= C[j * n + k] * C[j * n + k];
+    C[j * n + j] = std::sqrt(d);
+    for(int i = j + 1; i < n; ++i)
+    {
+      double v = h * sh.mass[i * n + j];
+      for(int k = 0; k < j; ++k)
+        v -= C[i * n + k] * C[j * n + k];
+      C[i * n + j] = v / C[j * n + j];
+    }
+  }
+  std::vector<double> Y((size_t)n * n), B((size_t)n * n), Q;
+  for(int col = 0; col < n; ++col)  // Y = C^{-1} L (forward substitution per column)
+    for(int i = 0; i < n; ++i)
+    {
+      double v = 0.5 * (L[i * n + col] + L[col * n + i]);
+      for(int k = 0; k < i; ++k)
+        v -= C[i * n + k] * Y[k * n + col];
+      Y[i * n + col] = v / C[i * n + i];
+    }
+  for(int row = 0; row < n; ++row)  // B = Y C^{-T}: solve C B^T = Y^T row by row
+    for(int i = 0; i < n; ++i)
+    {
+      double v = Y[row * n + i];
+      for(int k = 0; k < i; ++k)
+        v -= C[i * n + k] * B[row * n + k];
+      B[row * n + i] = v / C[i * n + i];
+    }
+  for(int i = 0; i < n; ++i)  // symmetrize the rounding
+    for(int j = i + 1; j < n; ++j)
+      B[i * n + j] = B[j * n + i] = 0.5 * (B[i * n + j] + B[j * n + i]);
+  symmetric_eigen(B, Q, n);
+  // positive semidefinite: a one-cell periodic direction has the constant mode (zero up to
+  // rounding); the sum over the three directions must stay positive (checked per cell)
+  double vmax = 0;
+  for(int a = 0; a < n; ++a)
+    vmax = std::max(vmax, std::fabs(B[a * n + a]));
+  for(int a = 0; a < n; ++a)
+  {
+    val[a] = B[a * n + a];
+    if(val[a] < -1e-12 * vmax)
+      EXADG_THROW(EXADG_ERR_INTERNAL, "1D cell block is not positive semidefinite");
+    val[a] = std::max(val[a], 0.0);
+  }
+  for(int col = 0; col < n; ++col)  // V = C^{-T} Q: back substitution per column
+    for(int i = n - 1; i >= 0; --i)
+    {
+      double v = Q[i * n + col];
+      for(int k = i + 1; k < n; ++k)
+        v -= C[k * n + i] * vec[k * n + col];
+      vec[i * n + col] = v / C[i * n + i];
+    }
+}
+}  // namespace
+
+template<typename Number>
+void LaplaceOperator<Number>::fast_diagonalization_tables(std::vector<double> & vec,
+                                                          std::vector<double> & val,
+                                                          std::vector<int> &    cfg) const
+{
+  if(!grid_->affine())
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "fast diagonalization needs an affine mesh");
+  BoxTopology const & t  = grid_->topology();
+  int const           n  = shape_.n;
+  int64_t const       nc = grid_->n_cells();
+  using Key              = std::tuple<int, int, int, int, double, double>;
+  std::map<Key, int>  index;
+  vec.clear();
+  val.clear();
+  cfg.assign((size_t)nc * 3, 0);
+  for(int64_t c = 0; c < nc; ++c)
+  {
+    int cc[3];
+    cell_coords(t, c, cc);
+    for(int d = 0; d < 3; ++d)
+    {
+      int64_t const nl = neighbor_id(t, cc, d, 0), nr = neighbor_id(t, cc, d, 1);
+      int const     tl = nl >= 0 ? -1 : t.bc[2 * d], tr = nr >= 0 ? -1 : t.bc[2 * d + 1];
+      bool const    self = nl == c;
+      double const  taul = geo_.tau_face[face_id(t, cc, d, 0)];
+      double const  taur = geo_.tau_face[face_id(t, cc, d, 1)];
+      Key const     key{d, tl, tr, self ? 1 : 0, taul, taur};
+      auto          it = index.find(key);
+      if(it == index.end())
+      {
+        int const id = (int)index.size();
+        it           = index.emplace(key, id).first;
+        vec.resize((size_t)(id + 1) * n * n);
+        val.resize((size_t)(id + 1) * n);
+        cell_line_eigen(shape_, grid_->h(d), tl, tr, self, taul, taur, &vec[(size_t)id * n * n],
+                        &val[(size_t)id * n]);
+      }
+      cfg[(size_t)c * 3 + d] = it->second;
+    }
+    double lmin = 0, lmax = 0;
+    for(int d = 0; d < 3; ++d)
+    {
+      double const * v = &val[(size_t)cfg[(size_t)c * 3 + d] * n];
+      lmin += *std::min_element(v, v + n);
+      lmax += *std::max_element(v, v + n);
+    }
+    if(!(lmin > 1e-12 * lmax))
+      EXADG_THROW(EXADG_ERR_ARGUMENT, "a cell block is singular (a one-cell box periodic in "
+                                      "every direction); block Jacobi is undefined");
+  }
+}
+
+template<typename Number>
+CellBlockInverse<Number>::CellBlockInverse(LaplaceOperator<Number> const & op, hipStream_t s)
+  : n_cells_(op.n_cells())
+{
+  m_   = (int)(op.m() / op.n_cells());
+  n1d_ = op.degree() + 1;
+  if(op.is_affine())
+  {
+    fdm_ = true;
+    std::vector<double> vec, val;
+    std::vector<int>    cfg;
+    op.fast_diagonalization_tables(vec, val, cfg);
+    vec_.upload(std::vector<Number>(vec.begin(), vec.end()));
+    val_.upload(std::vector<Number>(val.begin(), val.end()));
+    cfg_.upload(cfg);
+    return;
+  }
+  if(m_ > kMaxDenseBlock)
+    EXADG_THROW(EXADG_ERR_ARGUMENT,
+                "block Jacobi on curved meshes holds blocks up to 125 x 125 (degree <= 4)");
+  blocks_.resize((size_t)n_cells_ * m_ * m_);
+  op.calculate_inverse_block_diagonal(blocks_.data(), s);
+}
+
+template<typename Number>
+void CellBlockInverse<Number>::apply(BlockApplyArgs<Number> a, hipStream_t s) const
+{
+  a.m       = m_;
+  a.n_cells = n_cells_;
+  if(fdm_)
+  {
+    a.fdm_vec = vec_.data();
+    a.fdm_val = val_.data();
+    a.fdm_cfg = cfg_.data();
+    a.n1d     = n1d_;
+  }
+  else
+    a.blocks = blocks_.data();
+  hipError_t e = launch_block_apply<Number>(a, s);
+  if(e != hipSuccess)
+    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
 }
 
 template<typename Number>
@@ -501,6 +762,8 @@ std::unique_ptr<LaplaceOperatorBase> create_laplace_operator(exadg_mesh_desc con
 }
 
 template class LaplaceOperator<double>;
+template class CellBlockInverse<double>;
+template class CellBlockInverse<float>;
 template class LaplaceOperator<float>;
 
 }  // namespace Poisson

@@ -9,6 +9,7 @@
 #include "grid.h"
 #include "error_kernels.h"
 #include "guess_kernels.h"
+#include "block_kernels.h"
 #include "laplace_kernels.h"
 #include "shape_info.h"
 #include "vector_ops.h"
@@ -56,6 +57,31 @@ public:
 };
 
 template<typename Number>
+class LaplaceOperator;
+
+// The inverses of the cells' diagonal blocks A_KK (block Jacobi), in one of two exact forms:
+// on affine meshes by fast diagonalization (the block is the Kronecker sum
+// L_x (x) M (x) M + M (x) L_y (x) M + M (x) M (x) L_z of 1D operators that carry the cell's own face
+// terms; any degree), else as dense n^3 x n^3 inverses (degree <= 4).
+template<typename Number>
+class CellBlockInverse
+{
+public:
+  explicit CellBlockInverse(LaplaceOperator<Number> const & op, hipStream_t s);
+  // dst = f2 B (b - t) (variant 0; t may be null) or a Chebyshev update (a.variant 1, 2);
+  // the block fields of `a` are filled here
+  void apply(BlockApplyArgs<Number> a, hipStream_t s) const;
+  bool fast_diagonalization() const { return fdm_; }
+
+private:
+  bool                 fdm_     = false;
+  int                  m_       = 0, n1d_ = 0;
+  int64_t              n_cells_ = 0;
+  DeviceBuffer<Number> blocks_, vec_, val_;
+  DeviceBuffer<int>    cfg_;
+};
+
+template<typename Number>
 class LaplaceOperator : public LaplaceOperatorBase
 {
 public:
@@ -82,6 +108,11 @@ public:
   void calculate_block_diagonal(Number * blocks, hipStream_t s) const;
   // the inverses of those blocks (exact; blocks of at most 125 x 125, i.e. degree <= 4)
   void calculate_inverse_block_diagonal(Number * blocks, hipStream_t s) const;
+  // the 1D eigen-decompositions of the affine cell blocks for CellBlockInverse: per distinct
+  // configuration of a cell's two faces along a direction, V (n x n, row = node) and the
+  // eigenvalues l of L v = l h M v with V^T h M V = I; per cell three configuration indices
+  void fast_diagonalization_tables(std::vector<double> & vec, std::vector<double> & val,
+                                   std::vector<int> & cfg) const;
   // x_new = (1 + f1) x - f1 x_old (has_old) + f2 dinv (b - A x), the operator application and
   // the Chebyshev-Jacobi update in one kernel (PreconditionChebyshev's inner step,
   // chebyshev_smoother.h:37-227); x_new may alias x_old, not x

@@ -66,20 +66,15 @@ void cg_tridiagonal_extreme_eigenvalues(std::vector<double> const & alpha,
 template<typename Number>
 void JacobiData<Number>::apply(Number * z, Number const * r, int64_t n, hipStream_t s) const
 {
-  if(blocks == nullptr)
+  if(block == nullptr)
   {
     vec::scale(z, inv_diag, r, n, s);
     return;
   }
   BlockApplyArgs<Number> a;
-  a.blocks     = blocks;
-  a.b          = r;
-  a.dst        = z;
-  a.m          = m;
-  a.n_cells    = n / m;
-  hipError_t e = launch_block_apply<Number>(a, s);
-  if(e != hipSuccess)
-    EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+  a.b   = r;
+  a.dst = z;
+  block->apply(a, s);
 }
 
 namespace
@@ -167,10 +162,9 @@ public:
   {
     op_->calculate_inverse_diagonal(d, s);
   }
-  int  block_size() const override { return (int)(op_->m() / op_->n_cells()); }
-  void inverse_block_diagonal(Number * blocks, hipStream_t s) const override
+  std::unique_ptr<Poisson::CellBlockInverse<Number>> block_inverse(hipStream_t s) const override
   {
-    op_->calculate_inverse_block_diagonal(blocks, s);
+    return std::make_unique<Poisson::CellBlockInverse<Number>>(*op_, s);
   }
   void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const override
   {
@@ -214,26 +208,25 @@ void JacobiPreconditioner::vmult(double * dst, double const * src, hipStream_t s
 }
 
 BlockJacobiPreconditioner::BlockJacobiPreconditioner(Poisson::LaplaceOperator<double> const & op)
-  : op_(op), m_((int)(op.m() / op.n_cells()))
+  : op_(op)
 {
-  if(m_ > kMaxDenseBlock)
-    EXADG_THROW(EXADG_ERR_ARGUMENT, "block Jacobi holds blocks up to 125 x 125 (degree <= 4)");
-  blocks_.reinit(op.n_cells() * (int64_t)m_ * m_);
 }
 
 void BlockJacobiPreconditioner::update(hipStream_t s)
 {
-  op_.calculate_inverse_block_diagonal(blocks_.data(), s);
+  inv_ = std::make_unique<Poisson::CellBlockInverse<double>>(op_, s);
 }
 
 void BlockJacobiPreconditioner::vmult(double * dst, double const * src, hipStream_t s)
 {
   if(dst == src)
     EXADG_THROW(EXADG_ERR_ARGUMENT, "dst and src must not alias");
-  JacobiData<double> P;
-  P.blocks = blocks_.data();
-  P.m      = m_;
-  P.apply(dst, src, op_.m(), s);
+  if(!inv_)
+    EXADG_THROW(EXADG_ERR_NOT_READY, "the block-Jacobi preconditioner needs update()");
+  BlockApplyArgs<double> a;
+  a.b   = src;
+  a.dst = dst;
+  inv_->apply(a, s);
 }
 
 // ---------------------------------------------------------------- Q1 operator
@@ -542,13 +535,8 @@ template<typename Number>
 JacobiData<Number> ChebyshevSmoother<Number>::jacobi_data() const
 {
   JacobiData<Number> P;
-  if(block_m_ > 0)
-  {
-    P.blocks = blocks_.data();
-    P.m      = block_m_;
-  }
-  else
-    P.inv_diag = inv_diag_.data();
+  P.inv_diag = inv_diag_.data();
+  P.block    = block_.get();
   return P;
 }
 
@@ -561,7 +549,7 @@ void ChebyshevSmoother<Number>::setup(LevelOperator<Number> const & op, int degr
   degree       = degree_;
   int64_t const n = op.n();
   t_.reinit(n);
-  block_m_ = 0;
+  block_.reset();
   if(prec == SmootherPreconditioner::point_jacobi)
   {
     inv_diag_.reinit(n);
@@ -575,9 +563,7 @@ void ChebyshevSmoother<Number>::setup(LevelOperator<Number> const & op, int degr
                   prec == SmootherPreconditioner::block_jacobi
                     ? "Block Jacobi only implemented for DG!"
                     : "additive Schwarz on continuous levels is not implemented");
-    block_m_ = op.block_size();
-    blocks_.reinit(n / block_m_ * (int64_t)block_m_ * block_m_);
-    op.inverse_block_diagonal(blocks_.data(), s);
+    block_ = op.block_inverse(s);
   }
   // start vector of deal.II's eigenvalue estimate: (i mod 11) minus its mean (the smoother
   // carries no constraints object, so constrained entries are not zeroed);
@@ -662,18 +648,13 @@ void ChebyshevSmoother<Number>::vmult(Number *& x, Number *& x_alt, Number const
                                       hipStream_t s) const
 {
   int64_t const n = op_->n();
-  if(block_m_ > 0)
+  if(block_)
   {
     BlockApplyArgs<Number> a;
-    a.blocks     = blocks_.data();
-    a.b          = b;
-    a.dst        = x;
-    a.f2         = Number(1. / theta);
-    a.m          = block_m_;
-    a.n_cells    = n / block_m_;
-    hipError_t e = launch_block_apply<Number>(a, s);
-    if(e != hipSuccess)
-      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+    a.b   = b;
+    a.dst = x;
+    a.f2  = Number(1. / theta);
+    block_->apply(a, s);
   }
   else
     vec::chebyshev_update<Number>(x, nullptr, nullptr, b, nullptr, inv_diag_.data(), Number(0),
@@ -715,25 +696,20 @@ void ChebyshevSmoother<Number>::apply_step(Number * x_new, Number const * x, Num
                                            Number const * b, Number f1, Number f2, bool has_old,
                                            hipStream_t s) const
 {
-  if(block_m_ > 0)
+  if(block_)
   {
     // t = A x, then the block inverse with the Chebyshev update in its epilogue
     op_->vmult(t_.data(), x, s);
     BlockApplyArgs<Number> a;
-    a.blocks     = blocks_.data();
-    a.b          = b;
-    a.t          = t_.data();
-    a.x          = x;
-    a.x_old      = x_old;
-    a.dst        = x_new;
-    a.f1         = f1;
-    a.f2         = f2;
-    a.variant    = has_old ? 2 : 1;
-    a.m          = block_m_;
-    a.n_cells    = op_->n() / block_m_;
-    hipError_t e = launch_block_apply<Number>(a, s);
-    if(e != hipSuccess)
-      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+    a.b       = b;
+    a.t       = t_.data();
+    a.x       = x;
+    a.x_old   = x_old;
+    a.dst     = x_new;
+    a.f1      = f1;
+    a.f2      = f2;
+    a.variant = has_old ? 2 : 1;
+    block_->apply(a, s);
     return;
   }
   if(op_->chebyshev_step(x_new, x, x_old, b, inv_diag_.data(), f1, f2, has_old, s))

@@ -51,9 +51,8 @@ public:
   void update(hipStream_t s) override;
 
 private:
-  Poisson::LaplaceOperator<double> const & op_;
-  int                                      m_ = 0;
-  DeviceVector<double>                     blocks_;
+  Poisson::LaplaceOperator<double> const &                   op_;
+  std::unique_ptr<Poisson::CellBlockInverse<double>> inv_;
 };
 
 // point Jacobi (jacobi_preconditioner.h:33-82)
@@ -106,15 +105,14 @@ enum class SmootherPreconditioner : int
   additive_schwarz = 2   // on DG levels the block-Jacobi inverse (operator_base.cpp:2002-2009)
 };
 
-// point Jacobi (inv_diag) or block Jacobi (blocks: m x m inverse per cell) as a level
+// point Jacobi (inv_diag) or block Jacobi (the inverse cell blocks) as a level
 // preconditioner z = P r
 template<typename Number>
 struct JacobiData
 {
-  Number const * inv_diag = nullptr;
-  Number const * blocks   = nullptr;
-  int            m        = 0;
-  void           apply(Number * z, Number const * r, int64_t n, hipStream_t s) const;
+  Number const *                             inv_diag = nullptr;
+  Poisson::CellBlockInverse<Number> const * block    = nullptr;
+  void apply(Number * z, Number const * r, int64_t n, hipStream_t s) const;
 };
 
 struct MultigridData
@@ -163,9 +161,8 @@ public:
   virtual bool    continuous() const { return false; }
   // node numbering of a continuous level (null for DG levels)
   virtual CGTopology const * cg_topology() const { return nullptr; }
-  // cell block size n^3 of a DG level (0: no block diagonal) and the inverse cell blocks
-  virtual int  block_size() const { return 0; }
-  virtual void inverse_block_diagonal(Number * /*blocks*/, hipStream_t /*s*/) const
+  // the inverse cell blocks of a DG level (block Jacobi)
+  virtual std::unique_ptr<Poisson::CellBlockInverse<Number>> block_inverse(hipStream_t /*s*/) const
   {
     EXADG_THROW(EXADG_ERR_ARGUMENT, "Block Jacobi only implemented for DG!");
   }
@@ -268,11 +265,10 @@ private:
 
   JacobiData<Number> jacobi_data() const;
 
-  LevelOperator<Number> const * op_ = nullptr;
-  DeviceVector<Number>          inv_diag_;
-  DeviceVector<Number>          blocks_;  // block Jacobi: inverse cell blocks
-  int                           block_m_ = 0;
-  mutable DeviceVector<Number>  t_;
+  LevelOperator<Number> const *                      op_ = nullptr;
+  DeviceVector<Number>                               inv_diag_;
+  std::unique_ptr<Poisson::CellBlockInverse<Number>> block_;  // block Jacobi
+  mutable DeviceVector<Number>                       t_;
 };
 
 template<typename Number>

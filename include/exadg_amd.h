@@ -142,14 +142,14 @@ enum exadg_preconditioner_kind
   EXADG_PRECOND_NONE             = 0,
   EXADG_PRECOND_POINT_JACOBI     = 1,
   EXADG_PRECOND_MULTIGRID        = 2,
-  EXADG_PRECOND_BLOCK_JACOBI     = 3, /* exact inverse cell blocks, degree <= 4             */
+  EXADG_PRECOND_BLOCK_JACOBI     = 3, /* exact inverse cell blocks (curved: degree <= 4)    */
   EXADG_PRECOND_ADDITIVE_SCHWARZ = 4  /* on the DG space: the block-Jacobi inverse          */
 };
 
 /* PreconditionerSmoother (multigrid_parameters.h:161-167): the inner preconditioner of the
- * Chebyshev smoother. BLOCK_JACOBI and ADDITIVE_SCHWARZ need DG levels of degree <= 4 (the
- * reference refuses block Jacobi on continuous levels; additive Schwarz on continuous levels is
- * not implemented here). */
+ * Chebyshev smoother. BLOCK_JACOBI and ADDITIVE_SCHWARZ need DG levels, of degree <= 4 on
+ * curved meshes (the reference refuses block Jacobi on continuous levels; additive Schwarz on
+ * continuous levels is not implemented here). */
 enum exadg_smoother_preconditioner
 {
   EXADG_SMOOTHER_POINT_JACOBI     = 0,
@@ -224,8 +224,9 @@ typedef struct exadg_preconditioner exadg_preconditioner;
 /* The preconditioner factory of Poisson::Operator::setup_preconditioner_and_solver
  * (operator.cpp:306-398) for None / PointJacobi / BlockJacobi / AdditiveSchwarz / Multigrid;
  * `mg` is read for MULTIGRID only. BlockJacobi (block_jacobi_preconditioner.h) applies the exact
- * inverses of the cells' diagonal blocks A_KK, computed through the cell-based loop and inverted
- * once at setup (matrix-based variant, operator_base.cpp:789-815); degree <= 4.
+ * inverses of the cells' diagonal blocks A_KK: on affine meshes by fast diagonalization of
+ * their Kronecker form (any degree), on curved meshes as dense inverses computed through the
+ * cell-based loop at setup (matrix-based variant, operator_base.cpp:789-815; degree <= 4).
  * The operator must be FP64 (the outer solver's precision). Setup includes the diagonals,
  * the Chebyshev eigenvalue estimates and the coarse solver (PreconditionerBase::update).
  * Lifetime: a POINT_JACOBI, BLOCK_JACOBI or ADDITIVE_SCHWARZ preconditioner refers to `op` (as

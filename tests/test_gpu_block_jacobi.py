@@ -1,9 +1,10 @@
 """GPU parity of the block-Jacobi preconditioner and smoother (row f2) against the CPU oracle.
 
 * exadg_preconditioner_vmult with EXADG_PRECOND_BLOCK_JACOBI equals the exact solve with the
-  oracle's cell blocks A_KK (numpy) to 1e-12 (Cartesian k = 1..4, curved, periodic with a cell
-  that is its own neighbour, mixed boundaries); ADDITIVE_SCHWARZ gives the same vector on the DG
-  space; degree 5 is refused with EXADG_ERR_ARGUMENT.
+  oracle's cell blocks A_KK (numpy) to 1e-12: on affine meshes through fast diagonalization
+  (k = 1..8, periodic with a cell that is its own neighbour, mixed boundaries), on curved meshes
+  through the dense inverses (k <= 4); ADDITIVE_SCHWARZ gives the same vector on the DG space;
+  curved degree 5 is refused with EXADG_ERR_ARGUMENT.
 * Block-Jacobi PCG reaches the oracle's solution with the oracle's iteration count (+-1).
 * The multigrid V-cycle with the block-Jacobi Chebyshev smoother equals the oracle's to 1e-9 in
   FP64 (eigenvalue estimates to 1e-8) and to 1e-4 in FP32, and PCG with it reproduces the
@@ -32,6 +33,9 @@ CASES = [
     ("curved_k4", (2, 2, 2), 0.08, 4, (0, 0, 0), (D,) * 6, 4),
     ("periodic_self_k2", (1, 3, 2), 0.0, 1, (1, 0, 0), (D, D, D, N, D, D), 2),
     ("periodic_curved_k3", (2, 2, 2), 0.1, 3, (1, 1, 0), (D,) * 6, 3),
+    ("cart_k5", (2, 2, 1), 0.0, 1, (0, 0, 0), (D, N, D, D, D, D), 5),
+    ("cart_k6_periodic", (1, 2, 2), 0.0, 1, (1, 0, 0), (D,) * 6, 6),
+    ("cart_k8", (2, 1, 1), 0.0, 1, (0, 0, 0), (D, D, D, D, N, D), 8),
 ]
 
 
@@ -62,8 +66,9 @@ def test_block_jacobi_apply_matches_exact_block_solve(case, torch_cuda, product)
         assert err <= 1e-12
 
 
-def test_block_jacobi_refuses_degree_5(torch_cuda, product):
-    op = product.LaplaceOperator(product.mesh_desc((2, 2, 2)), product.bc_desc((D,) * 6), 5)
+def test_block_jacobi_refuses_curved_degree_5(torch_cuda, product):
+    op = product.LaplaceOperator(product.mesh_desc((2, 2, 2), deformation=0.1, mapping_degree=2),
+                                 product.bc_desc((D,) * 6), 5)
     with pytest.raises(product.ExaDGError) as e:
         product.Preconditioner(op, product.PRECOND_BLOCK_JACOBI)
     assert e.value.code == product.EXADG_ERR_ARGUMENT
