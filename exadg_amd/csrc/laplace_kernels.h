@@ -39,7 +39,9 @@ struct CartTable
   static constexpr int M = 0, K = cs_size(N), D0 = 2 * cs_size(N), D1 = D0 + N;
 };
 
-// Table layout (Number, SipgArgs::tab): general kernel: [S n*n | Dq n*n | d0 n | d1 n];
+// Table layout (Number, SipgArgs::tab): general kernel: [S n*n | Dq n*n | d0 n | d1 n], and in
+// device memory (`tables`) followed by [l(0) n | l(1) n | l'(0) n | l'(1) n] of the Lagrange
+// polynomials on the Gauss points;
 // Cartesian kernel: CartTable, on the unit interval, plus h per direction.
 template<typename Number>
 struct SipgArgs

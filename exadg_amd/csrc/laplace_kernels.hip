@@ -172,56 +172,51 @@ __device__ __forceinline__ void source_line(Number const * __restrict__ src, int
 //
 // Cell term by collocation at the Gauss points: u_q = (S x S x S) u, reference gradient
 // by the Gauss collocation derivative Dq, symmetric metric G = JxW J^{-1} J^{-T} per point,
-// then the transposed sweeps. Face terms per direction: traces and normal derivatives at
-// the face nodes of both sides, (S x S) to the face Gauss points, flux with the per-point
-// normal-derivative vectors a = J^{-1} n of both sides, transposed back and added to the
-// own lines.
-
-// Face directions handled per round of face phases: two (x and y together, then z) in FP32,
-// where the LDS has room for their 16 n^2 face arrays at full occupancy; one in FP64
-template<typename Number>
-constexpr int face_dirs_per_round()
-{
-  return sizeof(Number) == 4 ? 2 : 1;
-}
+// then the transposed sweeps. Face terms between the two: the own traces from the Gauss
+// values, the neighbour traces interpolated from the face nodes, flux with the per-point
+// normal-derivative vectors a = J^{-1} n of both sides, and the face tests added to the
+// Gauss-level residual before the transposed interpolation back to the nodes.
 
 template<int N, typename Number>
 constexpr int general_work_size()
 {
   // cell phase: the x and y gradients (the z one stays in registers); face phases: 8 trace
-  // arrays per direction of a round, which the flux coefficients then overwrite
-  // (one direction per round: 12 n^2, the flux results written at once, see general_faces)
-  constexpr int faces = face_dirs_per_round<Number>() == 1 ? 12 * N * N
-                                                           : 8 * face_dirs_per_round<Number>() * N * N;
-  return (2 * N * N * N > faces) ? 2 * N * N * N : faces;
+  // arrays per direction, two directions per round (x and y, then z)
+  return (2 * N * N * N > 16 * N * N) ? 2 * N * N * N : 16 * N * N;
 }
 
-// Face terms of the ND directions dirs[] (compile-time after unrolling) in one round of
-// barrier-separated steps; the face arrays of direction j live at W + 8 j N^2.
+// Face terms from the cell's values at the Gauss points. The own traces at the
+// face Gauss points are one dot product per end of the lane's d-line of U (the Gauss values, in
+// A at this point): u = sum_q l_q(0) U_q, du/dxi_d = sum_q l_q'(0) U_q with l_q the Lagrange
+// polynomials on the Gauss points; only the neighbour traces (at the face nodes) need the 2D
+// interpolation (S x S). The flux: per face point the traces and normal derivatives of both
+// sides (a = J^{-1} n from the per-face data), laplace_operator.h:180-197 with the mirror
+// values of weak_boundary_conditions.h on boundary faces; the tangential-gradient tests are
+// folded into the value test by Dq^T along the face lines.
+// The results end in four scalars per direction for the lane's d-line at Gauss indices (p, q):
+// fr[d] = (value test, normal-derivative test) at the lower end, then at the upper end; the
+// transposed derivative sweeps add l(0) fr[0] + l'(0) fr[1] + l(1) fr[2] + l'(1) fr[3] to that
+// line of the Gauss-level residual. Compared with face rounds after the cell phase on nodal
+// traces, this drops the 2D interpolation of the own-side traces and all transposed face
+// interpolations; the face arrays of direction j live at W + 8 j N^2.
 template<int N, typename Number, ApplyMode mode, int ND>
-__device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Number> const & a,
-                                              int64_t cell, int const * cc, bool active, int lt,
-                                              Number * A, Number * W,
-                                              Number const (&nbw)[3][2],
-                                              Number const (&nbdw)[3][2])
+__device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Number> const & a,
+                                            int64_t cell, int const * cc, bool active, int lt,
+                                            Number const * U, Number * W,
+                                            Number const (&nbw)[3][2], Number const (&nbdw)[3][2],
+                                            Number (&fr)[3][4])
 {
-  constexpr int  N2 = N * N, N3 = N * N * N;
-  Number const * S  = a.tables;
+  constexpr int  N2 = N * N;
   Number const * Dq = a.tables + N2;
   Number const * d0 = a.tables + 2 * N2;
   Number const * d1 = d0 + N;
+  Number const * S  = a.tables;
+  Number const * L0 = d1 + N;
+  Number const * L1 = L0 + N;
+  Number const * G0 = L1 + N;
+  Number const * G1 = G0 + N;
   int const      p = lt % N, q = lt / N;
-  // E_j[(s*4+c)*N2]: c = u-, du-/dxi_d, u+, du+/dxi_d. F4 reads u- and u+ across lanes but
-  // the derivatives only at the lane's own point, and produces four arrays per side: value,
-  // t1, t2 and d test coefficients. With one direction per round they are written at once, t1
-  // and d over the own-point derivative slots, value and t2 into 4 N^2 more; with two, the
-  // results wait in registers for a barrier and then overwrite E_j.
-  constexpr bool direct = ND == 1;
-  auto Rv  = [&](int j, int s) { return direct ? W + 8 * N2 + s * 2 * N2 : W + (8 * j + s * 4) * N2; };
-  auto Rt1 = [&](int j, int s) { return direct ? W + (s * 4 + 1) * N2 : W + (8 * j + s * 4 + 1) * N2; };
-  auto Rt2 = [&](int j, int s) { return direct ? W + 9 * N2 + s * 2 * N2 : W + (8 * j + s * 4 + 2) * N2; };
-  auto Rd  = [&](int j, int s) { return direct ? W + (s * 4 + 3) * N2 : W + (8 * j + s * 4 + 3) * N2; };
-  int type[ND][2];
+  int            type[ND][2];
 #pragma unroll
   for(int j = 0; j < ND; ++j)
 #pragma unroll
@@ -230,8 +225,7 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
       int64_t const nb = neighbor_id(a.topo, cc, dirs[j], s);
       type[j][s]       = nb >= 0 ? kInterior : a.topo.bc[2 * dirs[j] + s];
     }
-
-  // F1: traces at the face nodes
+  // F1: own traces at the face Gauss points, neighbour traces at the face nodes
   if(active)
 #pragma unroll
     for(int j = 0; j < ND; ++j)
@@ -240,45 +234,43 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
       Number *  E = W + 8 * j * N2;
       int       base, stride;
       line_geometry<N>(d, p, q, base, stride);
-      Number u[N];
-      source_line<N, Number, mode>(a.src, cell, base, stride, a.probe, u);
-      E[0 * N2 + lt] = u[0];
-      E[1 * N2 + lt] = dot_line<N>(d0, u);
-      E[4 * N2 + lt] = u[N - 1];
-      E[5 * N2 + lt] = dot_line<N>(d1, u);
+      Number v[N];
+      load_line<N>(U + base, stride, v);
+      E[0 * N2 + lt] = dot_line<N>(L0, v);
+      E[1 * N2 + lt] = dot_line<N>(G0, v);
+      E[4 * N2 + lt] = dot_line<N>(L1, v);
+      E[5 * N2 + lt] = dot_line<N>(G1, v);
+      Number nl = 0, dnl = 0, nr = 0, dnr = 0;
       if constexpr(mode != ApplyMode::diagonal)
       {
-        // neighbour traces, read at kernel start (their loads overlap the cell phases)
         if(type[j][0] == kInterior)
-        {
-          E[2 * N2 + lt] = nbw[d][0];
-          E[3 * N2 + lt] = nbdw[d][0];
-        }
+          nl = nbw[d][0], dnl = nbdw[d][0];
         if(type[j][1] == kInterior)
-        {
-          E[6 * N2 + lt] = nbw[d][1];
-          E[7 * N2 + lt] = nbdw[d][1];
-        }
+          nr = nbw[d][1], dnr = nbdw[d][1];
       }
       else
       {
         // probing the cell-local block: neighbours contribute zero, except a cell that is its
         // own periodic neighbour (one cell along d), whose other end then acts as neighbour
-        bool const sl = neighbor_id(a.topo, cc, d, 0) == cell;
-        bool const sr = neighbor_id(a.topo, cc, d, 1) == cell;
-        E[2 * N2 + lt] = sl ? u[N - 1] : Number(0);
-        E[3 * N2 + lt] = sl ? dot_line<N>(d1, u) : Number(0);
-        E[6 * N2 + lt] = sr ? u[0] : Number(0);
-        E[7 * N2 + lt] = sr ? dot_line<N>(d0, u) : Number(0);
+        Number u[N];
+        source_line<N, Number, mode>(a.src, cell, base, stride, a.probe, u);
+        if(neighbor_id(a.topo, cc, d, 0) == cell)
+          nl = u[N - 1], dnl = dot_line<N>(d1, u);
+        if(neighbor_id(a.topo, cc, d, 1) == cell)
+          nr = u[0], dnr = dot_line<N>(d0, u);
       }
+      E[2 * N2 + lt] = nl;
+      E[3 * N2 + lt] = dnl;
+      E[6 * N2 + lt] = nr;
+      E[7 * N2 + lt] = dnr;
     }
   __syncthreads();
-  // F2/F3: interpolate the 8 ND face arrays to the face Gauss points (S along t1, then t2)
+  // F2/F3: the neighbour arrays (slots 2, 3 of each side) to the face Gauss points
   if(active)
-    for(int task = lt; task < 8 * ND * N; task += N2)
+    for(int task = lt; task < 4 * ND * N; task += N2)
     {
-      Number *  arr = W + (task / N) * N2;
-      int const r   = task % N;
+      int const which = task / N, r = task % N;
+      Number *  arr   = W + (8 * (which >> 2) + 4 * ((which >> 1) & 1) + 2 + (which & 1)) * N2;
       Number    v[N], o[N];
       load_line<N>(arr + N * r, 1, v);
       mat_apply<N>(S, v, o);
@@ -286,18 +278,18 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
     }
   __syncthreads();
   if(active)
-    for(int task = lt; task < 8 * ND * N; task += N2)
+    for(int task = lt; task < 4 * ND * N; task += N2)
     {
-      Number *  arr = W + (task / N) * N2;
-      int const r   = task % N;
+      int const which = task / N, r = task % N;
+      Number *  arr   = W + (8 * (which >> 2) + 4 * ((which >> 1) & 1) + 2 + (which & 1)) * N2;
       Number    v[N], o[N];
       load_line<N>(arr + r, N, v);
       mat_apply<N>(S, v, o);
       store_line<N>(arr + r, N, o);
     }
   __syncthreads();
-  // F4: fluxes at the 2 x N^2 face quadrature points of each direction (side s, point lt),
-  // kept in registers until every lane has read E, then written over it
+  // F4: fluxes at the face quadrature points (side s, point lt), held in registers until
+  // every lane has read E
   Number res[ND][2][4];
   if(active)
 #pragma unroll
@@ -311,8 +303,6 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
         if(type[j][s] == EXADG_BC_NEUMANN)
         {  // homogeneous Neumann: u+ = u-, du+/dn = -du-/dn -> no contribution
           res[j][s][0] = res[j][s][1] = res[j][s][2] = res[j][s][3] = Number(0);
-          if constexpr(direct)
-            Rv(j, s)[qq] = Rt1(j, s)[qq] = Rt2(j, s)[qq] = Rd(j, s)[qq] = Number(0);
           continue;
         }
         int64_t const  f  = face_id(a.topo, cc, d, s);
@@ -325,8 +315,8 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
           aK[c]           = s ? am : -ap;
           aN[c]           = s ? ap : -am;
         }
-        Number const   w  = fd[6 * N2 + qq];
-        Number const * Es = W + 8 * j * N2 + s * 4 * N2;
+        Number const   w   = fd[6 * N2 + qq];
+        Number const * Es  = W + 8 * j * N2 + s * 4 * N2;
         Number         gt1 = 0, gt2 = 0;
 #pragma unroll
         for(int m = 0; m < N; ++m)
@@ -361,31 +351,18 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
         res[j][s][1]       = sigma * aK[t1];
         res[j][s][2]       = sigma * aK[t2];
         res[j][s][3]       = sigma * aK[d];
-        if constexpr(direct)
-        {
-          Rv(j, s)[qq]  = res[j][s][0];
-          Rt1(j, s)[qq] = res[j][s][1];
-          Rt2(j, s)[qq] = res[j][s][2];
-          Rd(j, s)[qq]  = res[j][s][3];
-        }
       }
   __syncthreads();
-  if constexpr(!direct)
-  {
-    if(active)
+  if(active)
 #pragma unroll
-      for(int j = 0; j < ND; ++j)
+    for(int j = 0; j < ND; ++j)
 #pragma unroll
-        for(int s = 0; s < 2; ++s)
-        {
-          Rv(j, s)[lt]  = res[j][s][0];
-          Rt1(j, s)[lt] = res[j][s][1];
-          Rt2(j, s)[lt] = res[j][s][2];
-          Rd(j, s)[lt]  = res[j][s][3];
-        }
-    __syncthreads();
-  }
-  // F5: tangential gradient test functions folded into the value test (Dq^T)
+      for(int s = 0; s < 2; ++s)
+#pragma unroll
+        for(int c = 0; c < 4; ++c)
+          W[(8 * j + 4 * s + c) * N2 + lt] = res[j][s][c];
+  __syncthreads();
+  // F5: tangential gradient tests folded into the value test (Dq^T); this lane's results
   if(active)
 #pragma unroll
     for(int j = 0; j < ND; ++j)
@@ -393,76 +370,25 @@ __device__ __forceinline__ void general_faces(int const (&dirs)[ND], SipgArgs<Nu
       for(int s = 0; s < 2; ++s)
       {
         int const      qq = lt, q1 = qq % N, q2 = qq / N;
-        Number *       rv = Rv(j, s);
-        Number const * r1 = Rt1(j, s);
-        Number const * r2 = Rt2(j, s);
-        Number         acc = rv[qq];
+        Number const * R  = W + (8 * j + 4 * s) * N2;
+        Number         acc = R[qq];
 #pragma unroll
         for(int m = 0; m < N; ++m)
-          acc += Dq[m * N + q1] * r1[m + N * q2] + Dq[m * N + q2] * r2[q1 + N * m];
-        rv[qq] = acc;
+          acc += Dq[m * N + q1] * R[N2 + m + N * q2] + Dq[m * N + q2] * R[2 * N2 + q1 + N * m];
+        fr[dirs[j]][2 * s]     = acc;
+        fr[dirs[j]][2 * s + 1] = R[3 * N2 + qq];
       }
-  __syncthreads();
-  // F6/F7: (S x S)^T back to the face nodes for the value and normal-derivative tests
-  if(active)
-    for(int task = lt; task < 4 * ND * N; task += N2)
-    {
-      int const which = task / N, r = task % N, j = which >> 2, w4 = which & 3;
-      Number *  arr   = (w4 & 1) ? Rd(j, w4 >> 1) : Rv(j, w4 >> 1);
-      Number    v[N], o[N];
-      load_line<N>(arr + N * r, 1, v);
-      mat_apply_T<N>(S, v, o);
-      store_line<N>(arr + N * r, 1, o);
-    }
-  __syncthreads();
-  if(active)
-    for(int task = lt; task < 4 * ND * N; task += N2)
-    {
-      int const which = task / N, r = task % N, j = which >> 2, w4 = which & 3;
-      Number *  arr   = (w4 & 1) ? Rd(j, w4 >> 1) : Rv(j, w4 >> 1);
-      Number    v[N], o[N];
-      load_line<N>(arr + r, N, v);
-      mat_apply_T<N>(S, v, o);
-      store_line<N>(arr + r, N, o);
-    }
-  __syncthreads();
-  // F8: add to the own d-lines, one direction at a time (lines of two directions share DoFs)
-#pragma unroll
-  for(int j = 0; j < ND; ++j)
-  {
-    if(active)
-    {
-      int base, stride;
-      line_geometry<N>(dirs[j], p, q, base, stride);
-      Number v[N];
-      load_line<N>(A + base, stride, v);
-#pragma unroll
-      for(int s = 0; s < 2; ++s)
-      {
-        if(type[j][s] == EXADG_BC_NEUMANN)
-          continue;
-        Number const   rv = Rv(j, s)[lt], rd = Rd(j, s)[lt];
-        Number const * dd = s ? d1 : d0;
-#pragma unroll
-        for(int i = 0; i < N; ++i)
-          v[i] += dd[i] * rd;
-        v[s ? N - 1 : 0] += rv;
-      }
-      store_line<N>(A + base, stride, v);
-    }
-    __syncthreads();
-  }
+  __syncthreads();  // W is reused next
 }
 
 // Minimum waves per SIMD asked of the compiler for the general kernel, matched to what the
-// LDS allows: FP64 at n = 6 four (128 VGPRs, a 12-byte spill, measured 8% faster than three)
-// and at n = 3 five (96 VGPRs, no spill, 5% faster than four; at n = 4 five spill and are
-// slower); FP32 at n = 4..6 seven (<= 72 VGPRs, no spill, 5-6% faster than six), at n = 7 six
+// LDS allows: FP64 at n = 6 three and at n = 3 four; FP32 at n = 4..6 seven (<= 72 VGPRs, a
+// 20-byte spill at n = 6 that measures no slower than six waves without it), at n = 7 six
 template<int N, typename Number>
 constexpr int general_min_waves()
 {
   if constexpr(sizeof(Number) == 8)
-    return N == 6 ? 4 : N == 3 ? 5 : 1;
+    return N == 6 ? 3 : N == 3 ? 4 : 1;
   else
     return (N >= 4 && N <= 6) ? 7 : N == 7 ? 6 : 1;
 }
@@ -558,6 +484,24 @@ __global__ void __launch_bounds__(256)
     mat_apply<N>(Dq, o, gz_line);
   }
   __syncthreads();
+  // the face terms from the Gauss values in A: four result scalars per direction per lane, added
+  // by the transposed derivative sweeps below
+  Number fr[3][4] = {};
+  {
+    int const xy[2] = {0, 1}, z[1] = {2};
+    gauss_faces<N, Number, mode, 2>(xy, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+    gauss_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+  }
+  Number const * L0 = a.tables + 2 * N2 + 2 * N;
+  Number const * L1 = L0 + N;
+  Number const * G0 = L1 + N;
+  Number const * G1 = G0 + N;
+  // line o of the Gauss-level residual += the face tests of the lane's d-line
+  auto add_face_tests = [&](Number(&o)[N], Number const(&f)[4]) {
+#pragma unroll
+    for(int i = 0; i < N; ++i)
+      o[i] += L0[i] * f[0] + G0[i] * f[1] + L1[i] * f[2] + G1[i] * f[3];
+  };
   if(active)
   {
     Number v[N], g[N];
@@ -592,6 +536,7 @@ __global__ void __launch_bounds__(256)
     Number v[N], o[N];
     load_line<N>(Gx + N * (p + N * q), 1, v);
     mat_apply_T<N>(Dq, v, o);
+    add_face_tests(o, fr[0]);
     store_line<N>(A + N * (p + N * q), 1, o);
   }
   __syncthreads();
@@ -600,6 +545,7 @@ __global__ void __launch_bounds__(256)
     Number v[N], o[N], w[N];
     load_line<N>(Gy + p + N2 * q, N, v);
     mat_apply_T<N>(Dq, v, o);
+    add_face_tests(o, fr[1]);
     load_line<N>(A + p + N2 * q, N, w);
 #pragma unroll
     for(int i = 0; i < N; ++i)
@@ -611,6 +557,7 @@ __global__ void __launch_bounds__(256)
   {
     Number o[N], w[N];
     mat_apply_T<N>(Dq, gz_line, o);
+    add_face_tests(o, fr[2]);
     load_line<N>(A + p + N * q, N2, w);
 #pragma unroll
     for(int i = 0; i < N; ++i)
@@ -635,22 +582,6 @@ __global__ void __launch_bounds__(256)
     store_line<N>(A + N * (p + N * q), 1, o);
   }
   __syncthreads();
-
-  if constexpr(face_dirs_per_round<Number>() == 2)
-  {
-    int const xy[2] = {0, 1}, z[1] = {2};
-    general_faces<N, Number, mode, 2>(xy, a, cell, cc, active, lt, A, W, nbw, nbdw);
-    general_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw);
-  }
-  else
-  {
-#pragma unroll
-    for(int d = 0; d < 3; ++d)
-    {
-      int const one[1] = {d};
-      general_faces<N, Number, mode, 1>(one, a, cell, cc, active, lt, A, W, nbw, nbdw);
-    }
-  }
 
   if(active)
   {

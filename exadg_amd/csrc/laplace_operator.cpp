@@ -80,6 +80,18 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
   append(plain, curved ? shape_.Dq : shape_.stiff);
   append(plain, shape_.d0);
   append(plain, shape_.d1);
+  if(curved)
+  {
+    // Lagrange polynomials on the Gauss points at the cell ends: values l_q(0), l_q(1), then
+    // derivatives l_q'(0), l_q'(1) (the general kernel's face traces from Gauss values)
+    std::vector<double> v0(n), v1(n), g0(n), g1(n);
+    lagrange_basis(shape_.qp, 0.0, v0.data(), g0.data());
+    lagrange_basis(shape_.qp, 1.0, v1.data(), g1.data());
+    append(plain, v0);
+    append(plain, v1);
+    append(plain, g0);
+    append(plain, g1);
+  }
   if((int)tab.size() > SipgArgs<Number>::kMaxTable)
     EXADG_THROW(EXADG_ERR_INTERNAL, "1D tables exceed the kernel-argument table");
   tab_.assign(tab.begin(), tab.end());
