@@ -42,12 +42,15 @@ __device__ __forceinline__ void mat_apply(Number const * __restrict__ M, Number 
   }
 }
 
-// out = A in for a centrosymmetric A stored as CartTable expects (even-odd form for n >= 5)
-template<int N, typename Number>
-__device__ __forceinline__ void cs_apply(Number const * __restrict__ T, Number const (&in)[N],
+// out = A in for a centrosymmetric (Skew = false) or skew-centrosymmetric (Skew = true) A,
+// stored in even-odd form when Packed (see cs_packed / gen_packed), else as the plain row-major
+// matrix. Centro: out_i = E_i + O_i, out_{n-1-i} = E_i - O_i, middle output (odd n) from the
+// even part; skew: out_{n-1-i} = O_i - E_i and the middle output from the odd part.
+template<int N, bool Packed, bool Skew, typename Number>
+__device__ __forceinline__ void eo_apply(Number const * __restrict__ T, Number const (&in)[N],
                                          Number (&out)[N])
 {
-  if constexpr(!cs_packed(N))
+  if constexpr(!Packed)
     mat_apply<N>(T, in, out);
   else
   {
@@ -75,33 +78,42 @@ __device__ __forceinline__ void cs_apply(Number const * __restrict__ T, Number c
       for(int j = 0; j < m; ++j)
         O += Ao[i * m + j] * o[j];
       out[i]         = E + O;
-      out[N - 1 - i] = E - O;
+      out[N - 1 - i] = Skew ? O - E : E - O;
     }
     if constexpr(N % 2 == 1)
     {
       Number s = 0;
+      if constexpr(Skew)
+      {
 #pragma unroll
-      for(int j = 0; j < me; ++j)
-        s += Am[j] * e[j];
+        for(int j = 0; j < m; ++j)
+          s += Am[j] * o[j];
+      }
+      else
+      {
+#pragma unroll
+        for(int j = 0; j < me; ++j)
+          s += Am[j] * e[j];
+      }
       out[m] = s;
     }
   }
 }
 
-// out[j] = sum_i M[i*N+j] in[i]
+// the Cartesian kernel's centrosymmetric 1D products (CartTable)
 template<int N, typename Number>
-__device__ __forceinline__ void mat_apply_T(Number const * __restrict__ M, Number const (&in)[N],
-                                            Number (&out)[N])
+__device__ __forceinline__ void cs_apply(Number const * __restrict__ T, Number const (&in)[N],
+                                         Number (&out)[N])
 {
-#pragma unroll
-  for(int j = 0; j < N; ++j)
-  {
-    Number s = 0;
-#pragma unroll
-    for(int i = 0; i < N; ++i)
-      s += M[i * N + j] * in[i];
-    out[j] = s;
-  }
+  eo_apply<N, cs_packed(N), false>(T, in, out);
+}
+
+// the general kernel's sweeps (GenTable): S and S^T centrosymmetric, Dq and Dq^T skew
+template<int N, bool Skew, typename Number>
+__device__ __forceinline__ void gen_apply(Number const * __restrict__ T, Number const (&in)[N],
+                                          Number (&out)[N])
+{
+  eo_apply<N, gen_packed(N), Skew>(T, in, out);
 }
 
 template<int N, typename Number>
@@ -207,14 +219,15 @@ __device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Numb
                                             Number (&fr)[3][4])
 {
   constexpr int  N2 = N * N;
-  Number const * Dq = a.tables + N2;
-  Number const * d0 = a.tables + 2 * N2;
-  Number const * d1 = d0 + N;
-  Number const * S  = a.tables;
-  Number const * L0 = d1 + N;
-  Number const * L1 = L0 + N;
-  Number const * G0 = L1 + N;
-  Number const * G1 = G0 + N;
+  using T             = GenTable<N>;
+  Number const * Dq = a.tables + T::DQ;
+  Number const * d0 = a.tables + T::D0;
+  Number const * d1 = a.tables + T::D1;
+  Number const * SE = a.tables + T::SE;
+  Number const * L0 = a.tables + T::L0;
+  Number const * L1 = a.tables + T::L1;
+  Number const * G0 = a.tables + T::G0;
+  Number const * G1 = a.tables + T::G1;
   int const      p = lt % N, q = lt / N;
   int            type[ND][2];
 #pragma unroll
@@ -273,7 +286,7 @@ __device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Numb
       Number *  arr   = W + (8 * (which >> 2) + 4 * ((which >> 1) & 1) + 2 + (which & 1)) * N2;
       Number    v[N], o[N];
       load_line<N>(arr + N * r, 1, v);
-      mat_apply<N>(S, v, o);
+      gen_apply<N, false>(SE, v, o);
       store_line<N>(arr + N * r, 1, o);
     }
   __syncthreads();
@@ -284,7 +297,7 @@ __device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Numb
       Number *  arr   = W + (8 * (which >> 2) + 4 * ((which >> 1) & 1) + 2 + (which & 1)) * N2;
       Number    v[N], o[N];
       load_line<N>(arr + r, N, v);
-      mat_apply<N>(S, v, o);
+      gen_apply<N, false>(SE, v, o);
       store_line<N>(arr + r, N, o);
     }
   __syncthreads();
@@ -408,8 +421,10 @@ __global__ void __launch_bounds__(256)
   Number *       A      = smem + (active ? slot : 0) * (N3 + WS);
   Number *       W      = A + N3;
   Number *       Gx = W, *Gy = W + N3;
-  Number const * S  = a.tables;
-  Number const * Dq = a.tables + N2;
+  Number const * SE  = a.tables + GenTable<N>::SE;
+  Number const * DE  = a.tables + GenTable<N>::DE;
+  Number const * STE = a.tables + GenTable<N>::STE;
+  Number const * DTE = a.tables + GenTable<N>::DTE;
   int const      p = lt % N, q = lt / N;
   int            cc[3] = {0, 0, 0};
   if(active)
@@ -422,8 +437,8 @@ __global__ void __launch_bounds__(256)
   if constexpr(mode != ApplyMode::diagonal)
     if(active)
     {
-      Number const * d0 = a.tables + 2 * N2;
-      Number const * d1 = d0 + N;
+      Number const * d0 = a.tables + GenTable<N>::D0;
+      Number const * d1 = a.tables + GenTable<N>::D1;
 #pragma unroll
       for(int d = 0; d < 3; ++d)
       {
@@ -460,7 +475,7 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], o[N];
     load_line<N>(A + N * (p + N * q), 1, v);
-    mat_apply<N>(S, v, o);
+    gen_apply<N, false>(SE, v, o);
     store_line<N>(A + N * (p + N * q), 1, o);
   }
   __syncthreads();
@@ -468,7 +483,7 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], o[N];
     load_line<N>(A + p + N2 * q, N, v);
-    mat_apply<N>(S, v, o);
+    gen_apply<N, false>(SE, v, o);
     store_line<N>(A + p + N2 * q, N, o);
   }
   __syncthreads();
@@ -479,9 +494,9 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], o[N];
     load_line<N>(A + p + N * q, N2, v);
-    mat_apply<N>(S, v, o);
+    gen_apply<N, false>(SE, v, o);
     store_line<N>(A + p + N * q, N2, o);
-    mat_apply<N>(Dq, o, gz_line);
+    gen_apply<N, true>(DE, o, gz_line);
   }
   __syncthreads();
   // the face terms from the Gauss values in A: four result scalars per direction per lane, added
@@ -492,10 +507,10 @@ __global__ void __launch_bounds__(256)
     gauss_faces<N, Number, mode, 2>(xy, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
     gauss_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
   }
-  Number const * L0 = a.tables + 2 * N2 + 2 * N;
-  Number const * L1 = L0 + N;
-  Number const * G0 = L1 + N;
-  Number const * G1 = G0 + N;
+  Number const * L0 = a.tables + GenTable<N>::L0;
+  Number const * L1 = a.tables + GenTable<N>::L1;
+  Number const * G0 = a.tables + GenTable<N>::G0;
+  Number const * G1 = a.tables + GenTable<N>::G1;
   // line o of the Gauss-level residual += the face tests of the lane's d-line
   auto add_face_tests = [&](Number(&o)[N], Number const(&f)[4]) {
 #pragma unroll
@@ -506,10 +521,10 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], g[N];
     load_line<N>(A + N * (p + N * q), 1, v);
-    mat_apply<N>(Dq, v, g);
+    gen_apply<N, true>(DE, v, g);
     store_line<N>(Gx + N * (p + N * q), 1, g);
     load_line<N>(A + p + N2 * q, N, v);
-    mat_apply<N>(Dq, v, g);
+    gen_apply<N, true>(DE, v, g);
     store_line<N>(Gy + p + N2 * q, N, g);
   }
   __syncthreads();
@@ -535,7 +550,7 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], o[N];
     load_line<N>(Gx + N * (p + N * q), 1, v);
-    mat_apply_T<N>(Dq, v, o);
+    gen_apply<N, true>(DTE, v, o);
     add_face_tests(o, fr[0]);
     store_line<N>(A + N * (p + N * q), 1, o);
   }
@@ -544,7 +559,7 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], o[N], w[N];
     load_line<N>(Gy + p + N2 * q, N, v);
-    mat_apply_T<N>(Dq, v, o);
+    gen_apply<N, true>(DTE, v, o);
     add_face_tests(o, fr[1]);
     load_line<N>(A + p + N2 * q, N, w);
 #pragma unroll
@@ -556,13 +571,13 @@ __global__ void __launch_bounds__(256)
   if(active)
   {
     Number o[N], w[N];
-    mat_apply_T<N>(Dq, gz_line, o);
+    gen_apply<N, true>(DTE, gz_line, o);
     add_face_tests(o, fr[2]);
     load_line<N>(A + p + N * q, N2, w);
 #pragma unroll
     for(int i = 0; i < N; ++i)
       w[i] += o[i];
-    mat_apply_T<N>(S, w, o);
+    gen_apply<N, false>(STE, w, o);
     store_line<N>(A + p + N * q, N2, o);
   }
   __syncthreads();
@@ -570,7 +585,7 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], o[N];
     load_line<N>(A + p + N2 * q, N, v);
-    mat_apply_T<N>(S, v, o);
+    gen_apply<N, false>(STE, v, o);
     store_line<N>(A + p + N2 * q, N, o);
   }
   __syncthreads();
@@ -578,7 +593,7 @@ __global__ void __launch_bounds__(256)
   {
     Number v[N], o[N];
     load_line<N>(A + N * (p + N * q), 1, v);
-    mat_apply_T<N>(S, v, o);
+    gen_apply<N, false>(STE, v, o);
     store_line<N>(A + N * (p + N * q), 1, o);
   }
   __syncthreads();

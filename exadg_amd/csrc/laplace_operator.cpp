@@ -14,20 +14,24 @@ namespace Poisson
 {
 namespace
 {
-// The even-odd form of a centrosymmetric n x n matrix (row-major) that the Cartesian kernel
-// reads (CartTable / cs_size); the plain matrix for n < 5. Throws if A is not
-// centrosymmetric, which it is for the 1D mass and stiffness on symmetric nodes.
-std::vector<double> centrosymmetric_pack(std::vector<double> const & A, int n)
+// The even-odd form of a centrosymmetric (skew = false) or skew-centrosymmetric (skew = true)
+// n x n matrix (row-major) that the kernels read (CartTable, GenTable, eo_size); the plain
+// matrix unless packed. Throws if A lacks the symmetry, which holds for the 1D mass and stiffness
+// on symmetric nodes (centro), the Gauss interpolation S (centro) and the Gauss collocation
+// derivative (skew).
+std::vector<double> even_odd_pack(std::vector<double> const & A, int n, bool packed, bool skew)
 {
-  if(!cs_packed(n))
+  if(!packed)
     return A;
   double amax = 0;
   for(double v : A)
     amax = std::max(amax, std::abs(v));
+  double const sign = skew ? -1.0 : 1.0;
   for(int i = 0; i < n; ++i)
     for(int j = 0; j < n; ++j)
-      if(std::abs(A[i * n + j] - A[(n - 1 - i) * n + (n - 1 - j)]) > 1e-12 * amax)
-        EXADG_THROW(EXADG_ERR_INTERNAL, "1D matrix is not centrosymmetric");
+      if(std::abs(A[i * n + j] - sign * A[(n - 1 - i) * n + (n - 1 - j)]) > 1e-12 * amax)
+        EXADG_THROW(EXADG_ERR_INTERNAL, skew ? "1D matrix is not skew-centrosymmetric"
+                                             : "1D matrix is not centrosymmetric");
   int const           m = n / 2, me = m + n % 2;
   std::vector<double> t;
   for(int i = 0; i < m; ++i)
@@ -37,10 +41,19 @@ std::vector<double> centrosymmetric_pack(std::vector<double> const & A, int n)
     for(int j = 0; j < m; ++j)
       t.push_back(0.5 * (A[i * n + j] - A[i * n + n - 1 - j]));
   if(n % 2)
-    for(int j = 0; j <= m; ++j)
-      t.push_back(A[m * n + j]);
-  if((int)t.size() != cs_size(n))
+    for(int j = 0; j <= m; ++j)  // middle row: coefficients of e_j (centro) or o_j (skew)
+      t.push_back(skew ? (j < m ? A[m * n + j] : 0.0) : A[m * n + j]);
+  if((int)t.size() != eo_size(n, true))
     EXADG_THROW(EXADG_ERR_INTERNAL, "even-odd table size");
+  return t;
+}
+
+std::vector<double> transposed(std::vector<double> const & A, int n)
+{
+  std::vector<double> t((size_t)n * n);
+  for(int i = 0; i < n; ++i)
+    for(int j = 0; j < n; ++j)
+      t[j * n + i] = A[i * n + j];
   return t;
 }
 }  // namespace
@@ -71,8 +84,8 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
   }
   else
   {
-    append(tab, centrosymmetric_pack(shape_.mass, n));
-    append(tab, centrosymmetric_pack(shape_.stiff, n));
+    append(tab, even_odd_pack(shape_.mass, n, cs_packed(n), false));
+    append(tab, even_odd_pack(shape_.stiff, n, cs_packed(n), false));
   }
   append(tab, shape_.d0);
   append(tab, shape_.d1);
@@ -91,6 +104,15 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
     append(plain, v1);
     append(plain, g0);
     append(plain, g1);
+    // the sweep matrices S, Dq, S^T, Dq^T in even-odd form (GenTable)
+    bool const gp = gen_packed(n);
+    append(plain, even_odd_pack(shape_.S, n, gp, false));
+    append(plain, even_odd_pack(shape_.Dq, n, gp, true));
+    append(plain, even_odd_pack(transposed(shape_.S, n), n, gp, false));
+    append(plain, even_odd_pack(transposed(shape_.Dq, n), n, gp, true));
+    int const size = 2 * n * n + 6 * n + 4 * gen_size(n);
+    if((int)plain.size() != size)
+      EXADG_THROW(EXADG_ERR_INTERNAL, "general-kernel table size");
   }
   if((int)tab.size() > SipgArgs<Number>::kMaxTable)
     EXADG_THROW(EXADG_ERR_INTERNAL, "1D tables exceed the kernel-argument table");
