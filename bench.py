@@ -351,13 +351,17 @@ def main():
                    "kernel_ms_mean": round(statistics.mean(kernel_ms), 5),
                    "kernel_ms_median": round(statistics.median(kernel_ms), 5),
                    "kernel_ms_min": round(min(kernel_ms), 5)},
-        # the roofline of a streaming operator is HBM; what limits this kernel below it is
-        # instruction issue (VALU + LDS phases serialized with the loads), per the counters
+        # the roofline of a streaming operator is HBM; what holds this kernel below it is the
+        # latency of each workgroup's chain (loads, LDS staging, five dependent LDS phases,
+        # store), overlapped only partly across the resident workgroups: the VALU is busy ~40%
+        # of the time, half of each wave's life is spent in waits, and probes removing VALU
+        # work, barriers or LDS traffic each gain only a few percent
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "limiter": "instruction issue: VALU and LDS phases, not HBM bytes "
-                                "(traffic ~1.1x algorithmic)"},
+                     "limiter": "latency of the per-workgroup load -> LDS phases -> store chain, "
+                                "not HBM bytes (traffic ~1.1x algorithmic) nor VALU/LDS "
+                                "throughput"},
         "stream": stream_bw,
     }
     if not args.no_solve:
