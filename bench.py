@@ -191,9 +191,12 @@ def run_combined(ex, torch, args):
     t_gen = time.perf_counter() - t0
     net = ex.GuessNet(k, history=3, width=128, seed=1)
     log("training")
+    # the network starts as the extrapolation (zero output layer) and is reset to it if the
+    # trained parameters do not beat the extrapolation on the training steps
     losses, t_train = ps.train(ex, net, torch, op, P, Bv, args.combined_train, args.combined_epochs,
-                               lr=1e-3, loss=ex.GUESS_LOSS_RESIDUAL)
-    log(f"train loss {losses[0]:.4g} -> {losses[-1]:.4g}; A/B")
+                               lr=1e-3, loss=ex.GUESS_LOSS_RESIDUAL, from_extrapolation=True)
+    kept = net.evaluate(P, Bv, 0, args.combined_train, op=op, loss=ex.GUESS_LOSS_RESIDUAL)
+    log(f"train loss {losses[0]:.4g} -> {losses[-1]:.4g}, kept {kept:.4g}; A/B")
     s0, s1 = args.combined_train, args.combined_steps
     ab32 = ps.ab_test(ex, torch, op, prec32, net, P, Bv, s0, s1, arms=("extrapolation", "learned"))
     ab64 = ps.ab_test(ex, torch, op, prec64, None, P, Bv, s0, s1, arms=("zero", "extrapolation"))
@@ -206,7 +209,11 @@ def run_combined(ex, torch, args):
         "steps": args.combined_steps, "train_steps": s0, "setup_s": round(t_setup, 1),
         "generation": {"time_s": round(t_gen, 2), "mean_iterations": float(np.mean(its[3:]))},
         "training": {"epochs": args.combined_epochs, "time_s": round(t_train, 2),
-                     "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1]},
+                     "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1],
+                     "loss_kept_parameters": kept,
+                     "loss": "residual |A e|^2 of the guess per training step, divided by the "
+                             "extrapolation's; training starts from the extrapolation (zero "
+                             "output layer) and falls back to it unless the loss is below 1"},
         "ms_per_step": {"fp32_mg_learned": per_step(ab32["learned"]),
                         "fp32_mg_extrapolation": per_step(ab32["extrapolation"]),
                         "fp64_mg_extrapolation_plain": per_step(ab64["extrapolation"]),

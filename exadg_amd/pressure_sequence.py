@@ -114,17 +114,34 @@ def ab_test(ex, torch, op, prec, net, P, Bv, s_begin, s_end, rel_tol=1e-10,
     return out
 
 
+def zero_output_layer(net):
+    """Set the output layer (W3, b3) to zero: the network's correction vanishes and the guess is
+    exactly the extrapolation, i.e. a loss of 1 on every step."""
+    p = net.params()
+    p[-net.n_outputs * (net.width + 1):] = 0
+    net.set_params(p)
+
+
 def train(ex, net, torch, op, P, Bv, t_end, epochs, lr, loss=None, batch_cells=4096, seed=7,
-          halve_every=5):
+          halve_every=5, from_extrapolation=False):
     """Train on steps [0, t_end) of the FP64 sequence; one call per epoch so that the per-epoch
     losses are returned. The learning rate halves every `halve_every` epochs, and the
     parameters after the epoch with the lowest training loss are kept: Adam on the residual
     loss can collapse late in training (the network output falling to zero, loss 1) after a
-    good early epoch. Only training steps are looked at."""
+    good early epoch. Only training steps are looked at.
+
+    With `from_extrapolation` the output layer starts at zero (the first guess is the
+    extrapolation, loss 1, and the hidden layers keep their random start), and if the kept
+    parameters do not beat the extrapolation on the training steps (loss >= 1, evaluated
+    after training) the output layer is zeroed again, so the learned guess is never worse than
+    the extrapolation on the data it has seen. The normalized losses make 1 the extrapolation's
+    value for every loss kind."""
     if loss is None:
         loss = ex.GUESS_LOSS_ENERGY
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    if from_extrapolation:
+        zero_output_layer(net)
     losses, best, best_params = [], float("inf"), None
     for ep in range(epochs):
         rate = lr * 0.5 ** (ep // halve_every)
@@ -134,5 +151,9 @@ def train(ex, net, torch, op, P, Bv, t_end, epochs, lr, loss=None, batch_cells=4
             best, best_params = losses[-1], net.params()
     if best_params is not None and not losses[-1] <= best:
         net.set_params(best_params)
+    if from_extrapolation:
+        final = net.evaluate(P, Bv, 0, t_end, op=op, loss=loss)
+        if not final < 1.0:
+            zero_output_layer(net)
     torch.cuda.synchronize()
     return losses, time.perf_counter() - t0

@@ -157,6 +157,38 @@ def test_learned_guess_ab(torch_cuda, product):
     assert r["learned"]["fallbacks"] == 0
 
 
+def test_training_from_the_extrapolation_never_loses_to_it(torch_cuda, product):
+    """`train(from_extrapolation=True)`: the zeroed output layer makes the first guess the
+    extrapolation (loss 1 before any step); the kept parameters either beat it on the training
+    steps or the output layer is zeroed again, and then the guess equals the extrapolation to
+    rounding. With no epoch at all the second case is forced."""
+    from exadg_amd import pressure_sequence as ps
+    torch = torch_cuda
+    op, prec, P, Bv, its = _sequence(torch, product, 4, 2, 24)
+    loss = product.GUESS_LOSS_RESIDUAL
+    net = product.GuessNet(2, history=3, width=32, seed=2)
+    ps.zero_output_layer(net)
+    assert abs(net.evaluate(P, Bv, 0, 16, op=op, loss=loss) - 1.0) < 1e-6
+    losses, _ = ps.train(product, net, torch, op, P, Bv, 16, epochs=8, lr=1e-3, loss=loss,
+                         from_extrapolation=True)
+    kept = net.evaluate(P, Bv, 0, 16, op=op, loss=loss)
+    print("losses", losses, "kept", kept, flush=True)
+    assert kept < 1.0 or abs(kept - 1.0) < 1e-6
+    # no epoch: the output layer stays zero and the guess is the extrapolation
+    net2 = product.GuessNet(2, history=3, width=32, seed=9)
+    ps.train(product, net2, torch, op, P, Bv, 16, epochs=0, lr=1e-3, loss=loss,
+             from_extrapolation=True)
+    w3b3 = net2.params()[-net2.n_outputs * (net2.width + 1):]
+    assert float(w3b3.abs().max()) == 0.0
+    s = 20
+    hist = [P[s - 1], P[s - 2], P[s - 3]]
+    x0, xe = torch.empty_like(P[0]), torch.empty_like(P[0])
+    assert not net2.predict(hist, Bv[s], x0)
+    product.extrapolate(hist, xe)
+    torch.cuda.synchronize()
+    assert _rel(x0.cpu().numpy(), xe.cpu().numpy()) <= 1e-15
+
+
 def test_operator_loss_needs_operator(torch_cuda, product):
     torch = torch_cuda
     net = product.GuessNet(2, history=3, width=16, seed=1)
