@@ -148,23 +148,35 @@ void StructuredGrid::evaluate(int64_t c, std::vector<double> const & px,
     return;
   }
   // MappingQ: support points are push_forward of the chart-space GL points of the cell
-  // (vertices and all higher-order support points of a ChartManifold cell)
-  int const           m = (int)map_nodes_.size();
-  std::vector<double> X(3 * (size_t)m * m * m);
+  // (vertices and all higher-order support points of a ChartManifold cell). The deformation
+  // factor of push_forward is a product of one sine per direction, so the m^3 points need 3m
+  // sines; the factors are multiplied in push_forward's order, so the points are bit-identical
+  // to calling it per point.
+  int const                  m = (int)map_nodes_.size();
+  thread_local std::vector<double> X;
+  X.resize(3 * (size_t)m * m * m);
+  double pc[3][16], sn[3][16];
+  for(int e = 0; e < 3; ++e)
+    for(int i = 0; i < m; ++i)
+    {
+      pc[e][i] = mesh_.lower[e] + h_[e] * (cc[e] + map_nodes_[i]);
+      sn[e][i] = std::sin(mesh_.frequency * M_PI * (pc[e][i] - mesh_.lower[e]) /
+                          (mesh_.upper[e] - mesh_.lower[e]));
+    }
   for(int k = 0; k < m; ++k)
     for(int j = 0; j < m; ++j)
       for(int i = 0; i < m; ++i)
       {
-        double const p[3] = {mesh_.lower[0] + h_[0] * (cc[0] + map_nodes_[i]),
-                             mesh_.lower[1] + h_[1] * (cc[1] + map_nodes_[j]),
-                             mesh_.lower[2] + h_[2] * (cc[2] + map_nodes_[k])};
-        double xx[3];
-        push_forward(p, xx);
+        double       f = mesh_.deformation;
+        f *= sn[0][i];
+        f *= sn[1][j];
+        f *= sn[2][k];
         size_t const s = i + (size_t)m * (j + (size_t)m * k);
-        for(int e = 0; e < 3; ++e)
-          X[e * (size_t)m * m * m + s] = xx[e];
+        X[s]                         = pc[0][i] + f;
+        X[(size_t)m * m * m + s]     = pc[1][j] + f;
+        X[2 * (size_t)m * m * m + s] = pc[2][k] + f;
       }
-  std::vector<double> V[3], D[3];
+  thread_local std::vector<double> V[3], D[3];
   for(int d = 0; d < 3; ++d)
   {
     int const mp = (int)pts[d]->size();
@@ -173,7 +185,7 @@ void StructuredGrid::evaluate(int64_t c, std::vector<double> const & px,
     for(int a = 0; a < mp; ++a)
       lagrange_basis(map_nodes_, (*pts[d])[a], &V[d][(size_t)a * m], &D[d][(size_t)a * m]);
   }
-  std::vector<double> w1, w2;
+  thread_local std::vector<double> w1, w2;
   for(int e = 0; e < 3; ++e)
   {
     double const * Xe = &X[e * (size_t)m * m * m];
