@@ -10,6 +10,42 @@ namespace
 void contract3(int ma, int mb, int mc, int m, double const * A, double const * B, double const * C,
                double const * in, double * out, std::vector<double> & w1, std::vector<double> & w2)
 {
+  if(mc == 1 || mb == 1)
+  {
+    // a face normal to z or y: contract that single point first (m^3), then the plane (m^2 per
+    // output line) instead of carrying all m planes through the other contractions
+    bool const   zf = mc == 1;
+    int const    mo = zf ? mb : mc;           // the other tangential output count
+    double const * R  = zf ? C : B;            // the single row of the face direction
+    double const * T  = zf ? B : C;            // the tangential matrix of the other direction
+    w1.resize((size_t)m * m);
+    for(int o = 0; o < m; ++o)
+      for(int i = 0; i < m; ++i)
+      {
+        double s = 0;
+        for(int l = 0; l < m; ++l)
+          s += R[l] * (zf ? in[i + m * (o + m * l)] : in[i + m * (l + m * o)]);
+        w1[i + m * o] = s;
+      }
+    w2.resize((size_t)ma * m);
+    for(int o = 0; o < m; ++o)
+      for(int a = 0; a < ma; ++a)
+      {
+        double s = 0;
+        for(int i = 0; i < m; ++i)
+          s += A[a * m + i] * w1[i + m * o];
+        w2[a + ma * o] = s;
+      }
+    for(int b = 0; b < mo; ++b)
+      for(int a = 0; a < ma; ++a)
+      {
+        double s = 0;
+        for(int o = 0; o < m; ++o)
+          s += T[b * m + o] * w2[a + ma * o];
+        out[a + ma * b] = s;  // (a, b) with the face direction's single index 0
+      }
+    return;
+  }
   w1.resize((size_t)ma * m * m);
   w2.resize((size_t)ma * mb * m);
   for(int k = 0; k < m; ++k)
