@@ -17,7 +17,11 @@ enum class ApplyMode : int
   // x_new = (1 + f1) x - f1 x_old (if has_old) + f2 dinv (b - A x); x_old may alias dst
   chebyshev = 3,
   // residual in the epilogue: dst = rhs - A src (dst may alias rhs, not src)
-  residual = 4
+  residual = 4,
+  // residual and p-restriction in the epilogue: per cell
+  // dst_coarse += (R x R x R)(rhs - A src), R = P^T with P the n x nc prolongation matrix;
+  // dst is the coarse DG vector of the same cells (nc^3 values per cell)
+  residual_restrict = 5
 };
 
 // Centrosymmetric n x n matrices (A_ij = A_{n-1-i,n-1-j}, such as the 1D mass and stiffness
@@ -88,6 +92,9 @@ struct SipgArgs
   Number         f1      = 0;
   Number         f2      = 0;
   int            has_old = 0;
+  // residual_restrict mode: P (n x nc, row-major) and nc
+  Number const * restrict_P  = nullptr;
+  int            restrict_nc = 0;
 };
 
 // degree 1..8; mode as above. Returns hipError of the launch.

@@ -598,6 +598,59 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
+  if constexpr(mode == ApplyMode::residual_restrict)
+  {
+    // r = rhs - A u in A, then the three transposed interpolation sweeps x, y, z of the
+    // restriction through the workspace; the coarse cell's values are added to dst
+    int const      nc = a.restrict_nc;
+    Number const * P  = a.restrict_P;  // P[i nc + c]: fine node i, coarse node c
+    Number *       W1 = W;             // nc x N x N
+    Number *       W2 = W + N3;        // nc x nc x N
+    if(active)
+      for(int e = lt; e < N3; e += N2)
+        A[e] = a.rhs[cell * N3 + e] - A[e];
+    __syncthreads();
+    if(active)
+    {
+      int const j = lt % N, k = lt / N;  // x-line (j, k)
+      for(int c = 0; c < nc; ++c)
+      {
+        Number s = 0;
+#pragma unroll
+        for(int i = 0; i < N; ++i)
+          s += P[i * nc + c] * A[i + N * (j + N * k)];
+        W1[c + nc * (j + N * k)] = s;
+      }
+    }
+    __syncthreads();
+    if(active && lt < nc * N)
+    {
+      int const c0 = lt % nc, k = lt / nc;  // y-line (c0, k)
+      for(int c1 = 0; c1 < nc; ++c1)
+      {
+        Number s = 0;
+#pragma unroll
+        for(int j = 0; j < N; ++j)
+          s += P[j * nc + c1] * W1[c0 + nc * (j + N * k)];
+        W2[c0 + nc * (c1 + nc * k)] = s;
+      }
+    }
+    __syncthreads();
+    if(active && lt < nc * nc)
+    {
+      int const     c0 = lt % nc, c1 = lt / nc;  // z-line (c0, c1)
+      int64_t const nc3 = (int64_t)nc * nc * nc;
+      for(int c2 = 0; c2 < nc; ++c2)
+      {
+        Number s = 0;
+#pragma unroll
+        for(int k = 0; k < N; ++k)
+          s += P[k * nc + c2] * W2[c0 + nc * (c1 + nc * k)];
+        a.dst[cell * nc3 + c0 + nc * (c1 + nc * c2)] += s;
+      }
+    }
+    return;
+  }
   if(active)
   {
     if constexpr(mode == ApplyMode::diagonal)
@@ -908,6 +961,60 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
     cs_apply<N>(tz + CartTable<N>::M, v, outp);
   }
   __syncthreads();
+  if constexpr(mode == ApplyMode::residual_restrict)
+  {
+    // r = rhs - A u along the lane's z-line, restricted along z in registers, then along x and
+    // y through A and B (free after the barrier above); the coarse cell is added to dst
+    int const      nc = a.restrict_nc;
+    Number const * P  = a.restrict_P;  // P[i nc + c]: fine node i, coarse node c
+    if(active)
+    {
+      Number v[N], r[N];
+      load_line<N>(U + bz, L::PY, v);
+      size_t const g0 = (size_t)cell * N3 + lt;
+#pragma unroll
+      for(int k = 0; k < N; ++k)
+        r[k] = a.rhs[g0 + k * N2] - (outp[k] + v[k]);
+#pragma unroll
+      for(int c = 0; c < N; ++c)
+        if(c < nc)
+        {
+          Number s = 0;
+#pragma unroll
+          for(int k = 0; k < N; ++k)
+            s += P[k * nc + c] * r[k];
+          Abuf[p + N * q + N2 * c] = s;  // (x, y, c_z)
+        }
+    }
+    __syncthreads();
+    if(active && lt < N * nc)
+    {
+      int const y = lt % N, cz = lt / N;
+      for(int cx = 0; cx < nc; ++cx)
+      {
+        Number s = 0;
+#pragma unroll
+        for(int x = 0; x < N; ++x)
+          s += P[x * nc + cx] * Abuf[x + N * y + N2 * cz];
+        Bbuf[cx + nc * (y + N * cz)] = s;
+      }
+    }
+    __syncthreads();
+    if(active && lt < nc * nc)
+    {
+      int const     cx = lt % nc, cz = lt / nc;
+      int64_t const nc3 = (int64_t)nc * nc * nc;
+      for(int cy = 0; cy < nc; ++cy)
+      {
+        Number s = 0;
+#pragma unroll
+        for(int y = 0; y < N; ++y)
+          s += P[y * nc + cy] * Bbuf[cx + nc * (y + N * cz)];
+        a.dst[(int64_t)cell * nc3 + cx + nc * (cy + nc * cz)] += s;
+      }
+    }
+    return;
+  }
   // out = outp + the z-term held in U, along the z-lines
   if constexpr(mode == ApplyMode::diagonal)
   {
@@ -1029,6 +1136,10 @@ hipError_t launch_sipg_general(int degree, ApplyMode mode, SipgArgs<Number> cons
     case ApplyMode::diagonal: return dispatch_general<Number, ApplyMode::diagonal>(degree, a, s);
     case ApplyMode::chebyshev: return dispatch_general<Number, ApplyMode::chebyshev>(degree, a, s);
     case ApplyMode::residual: return dispatch_general<Number, ApplyMode::residual>(degree, a, s);
+    case ApplyMode::residual_restrict:
+      if(a.restrict_P == nullptr || a.restrict_nc < 1 || a.restrict_nc > degree + 1)
+        return hipErrorInvalidValue;
+      return dispatch_general<Number, ApplyMode::residual_restrict>(degree, a, s);
   }
   return hipErrorInvalidValue;
 }
@@ -1045,6 +1156,10 @@ hipError_t launch_sipg_cartesian(int degree, ApplyMode mode, SipgArgs<Number> co
     case ApplyMode::chebyshev:
       return dispatch_cartesian<Number, ApplyMode::chebyshev>(degree, a, s);
     case ApplyMode::residual: return dispatch_cartesian<Number, ApplyMode::residual>(degree, a, s);
+    case ApplyMode::residual_restrict:
+      if(a.restrict_P == nullptr || a.restrict_nc < 1 || a.restrict_nc > degree + 1)
+        return hipErrorInvalidValue;
+      return dispatch_cartesian<Number, ApplyMode::residual_restrict>(degree, a, s);
   }
   return hipErrorInvalidValue;
 }

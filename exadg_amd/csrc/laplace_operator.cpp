@@ -257,6 +257,22 @@ void LaplaceOperator<Number>::residual(Number * dst, Number const * src, Number 
 }
 
 template<typename Number>
+bool LaplaceOperator<Number>::residual_restrict(Number * coarse, Number const * src, Number const * b,
+                                                Number const * P, int nc, hipStream_t s) const
+{
+  if(coarse == nullptr || src == nullptr || b == nullptr || P == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  if(nc < 1 || nc > data_.degree + 1)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "coarse degree out of range");
+  SipgArgs<Number> a = make_args(coarse, src, 0);
+  a.rhs              = b;
+  a.restrict_P       = P;
+  a.restrict_nc      = nc;
+  launch(ApplyMode::residual_restrict, a, s);
+  return true;
+}
+
+template<typename Number>
 void LaplaceOperator<Number>::vmult(Number * dst, Number const * src, hipStream_t s) const
 {
   apply(ApplyMode::vmult, dst, src, 0, s);

@@ -121,6 +121,11 @@ public:
                       hipStream_t s) const;
   // dst = b - A src in one kernel (the residual in the vmult epilogue); dst may alias b
   void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const;
+  // coarse += (R x R x R)(b - A src) per cell in one kernel, R = P^T with P the (k+1) x nc
+  // prolongation matrix (device, row-major) to a DG level of degree nc-1 on the same cells;
+  // returns true (the fused form exists for both kernels)
+  bool residual_restrict(Number * coarse, Number const * src, Number const * b, Number const * P,
+                         int nc, hipStream_t s) const;
 
   void vmult(void * dst, void const * src, hipStream_t s) const override
   {

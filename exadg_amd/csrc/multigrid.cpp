@@ -170,6 +170,11 @@ public:
   {
     op_->residual(dst, src, b, s);
   }
+  bool residual_restrict(Number * coarse, Number const * src, Number const * b, Number const * P,
+                         int nc, hipStream_t s) const override
+  {
+    return op_->residual_restrict(coarse, src, b, P, nc, s);
+  }
   bool chebyshev_step(Number * x_new, Number const * x, Number const * x_old, Number const * b,
                       Number const * dinv, Number f1, Number f2, bool has_old,
                       hipStream_t s) const override
@@ -1184,8 +1189,16 @@ void MultigridPreconditioner<Number>::v_cycle(int l, hipStream_t s)
   Level &       L = levels_[l];
   int64_t const n = L.op->n();
   L.smoother.vmult(L.sol, L.sol_alt, L.defect.data(), s);
-  L.op->residual(L.t.data(), L.sol, L.defect.data(), s);  // t = defect - A sol
-  restrict_and_add(l, levels_[l - 1].defect.data(), L.t.data(), s);
+  // t = defect - A sol, restricted to the next level's defect: one fused pass for a DG
+  // p-transfer where the operator has it, else the residual kernel and the transfer
+  bool const fused = L.dg && L.transfer == 'p' &&
+                     L.op->residual_restrict(levels_[l - 1].defect.data(), L.sol, L.defect.data(),
+                                             L.P.data(), levels_[l - 1].degree + 1, s);
+  if(!fused)
+  {
+    L.op->residual(L.t.data(), L.sol, L.defect.data(), s);
+    restrict_and_add(l, levels_[l - 1].defect.data(), L.t.data(), s);
+  }
   v_cycle(l - 1, s);
   prolongate_and_add(l, L.sol, levels_[l - 1].sol, s);
   L.smoother.step(L.sol, L.sol_alt, L.defect.data(), s);

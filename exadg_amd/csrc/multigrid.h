@@ -169,6 +169,13 @@ public:
   // dst = b - A src; operators with a fused epilogue override this (the default applies A and
   // then updates, so it needs dst != b)
   virtual void residual(Number * dst, Number const * src, Number const * b, hipStream_t s) const;
+  // coarse += R (b - A src) with the DG p-restriction R = P^T per cell (P: (k+1) x nc) in one
+  // pass; false if this operator has no fused form
+  virtual bool residual_restrict(Number * /*coarse*/, Number const * /*src*/, Number const * /*b*/,
+                                 Number const * /*P*/, int /*nc*/, hipStream_t /*s*/) const
+  {
+    return false;
+  }
   // mask of constrained entries (1 constrained, 0 free) on the host; empty if none
   virtual std::vector<char> constrained_mask() const { return {}; }
   // x_new = (1+f1) x - f1 x_old (if has_old) + f2 dinv (b - A x) in one pass; false if this
