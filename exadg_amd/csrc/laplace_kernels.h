@@ -101,6 +101,16 @@ struct SipgArgs
 template<typename Number>
 hipError_t launch_sipg_general(int degree, ApplyMode mode, SipgArgs<Number> const & args,
                                hipStream_t stream);
+// Diagonal of the affine SIPG operator in closed form, one thread per DoF: the cell block is
+// L_x (x) h_y M (x) h_z M + h_x M (x) L_y (x) h_z M + h_x M (x) h_y M (x) L_z with the 1D
+// cell-local line operators L_d (the cell's own face terms, neighbours zero, or the cell's own
+// values across a periodic face of a one-cell direction), so its diagonal is a sum of three
+// products of 1D diagonals. diag1d: [M_ii | K_ii | d0 | d1] of the unit interval (4n doubles).
+template<typename Number>
+hipError_t launch_cartesian_diagonal(Number * diag, double const * diag1d, Number const * tau_face,
+                                     BoxTopology const & topo, double const (&h)[3], int n,
+                                     int64_t n_cells, hipStream_t stream);
+
 template<typename Number>
 hipError_t launch_sipg_cartesian(int degree, ApplyMode mode, SipgArgs<Number> const & args,
                                  hipStream_t stream);

@@ -1123,6 +1123,75 @@ hipError_t dispatch_cartesian(int degree, SipgArgs<Number> const & a, hipStream_
     default: return hipErrorInvalidValue;
   }
 }
+// diagonal entry i of the 1D cell-local line operator along a direction of width h: the line
+// Laplacian of the Cartesian kernel applied to the unit vector e_i, with the exterior trace of
+// weak_boundary_conditions.h on boundary faces (type_l/type_r EXADG_BC_*), zero across interior
+// faces, or the cell's own opposite end across the periodic faces of a one-cell direction (self)
+__device__ inline double line_diagonal(double const * t, int n, int i, double h, int type_l,
+                                       int type_r, bool self, double tau_l, double tau_r)
+{
+  double const * M = t;
+  double const * K = t + n;
+  double const * d0 = t + 2 * n;
+  double const * d1 = t + 3 * n;
+  (void)M;
+  double const u0 = i == 0 ? 1.0 : 0.0, un = i == n - 1 ? 1.0 : 0.0;
+  double const du0 = d0[i], du1 = d1[i];
+  double       wl = 0, dwl = 0, wr = 0, dwr = 0;
+  if(type_l == EXADG_BC_DIRICHLET)
+    wl = -u0, dwl = du0;
+  else if(type_l == EXADG_BC_NEUMANN)
+    wl = u0, dwl = -du0;
+  else if(self)
+    wl = un, dwl = du1;
+  if(type_r == EXADG_BC_DIRICHLET)
+    wr = -un, dwr = du1;
+  else if(type_r == EXADG_BC_NEUMANN)
+    wr = un, dwr = -du1;
+  else if(self)
+    wr = u0, dwr = du0;
+  double const jl = 0.5 * (u0 - wl), jr = 0.5 * (un - wr);
+  double       o  = (K[i] + d0[i] * jl - d1[i] * jr) / h;
+  if(i == 0)
+    o += 0.5 * (du0 + dwl) / h + 2.0 * tau_l * jl;
+  if(i == n - 1)
+    o += 2.0 * tau_r * jr - 0.5 * (du1 + dwr) / h;
+  return o;
+}
+
+template<typename Number>
+__global__ void __launch_bounds__(256)
+  cartesian_diagonal_kernel(Number * diag, double const * t, Number const * tau_face,
+                            BoxTopology topo, double h0, double h1, double h2, int n,
+                            int64_t n_dofs)
+{
+  int64_t const g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(g >= n_dofs)
+    return;
+  int const     n3   = n * n * n;
+  int64_t const cell = g / n3;
+  int const     e    = (int)(g - cell * n3);
+  int const     id[3] = {e % n, (e / n) % n, e / (n * n)};
+  int           cc[3];
+  cell_coords(topo, cell, cc);
+  double const h[3] = {h0, h1, h2};
+  double       L[3], m[3];
+#pragma unroll
+  for(int d = 0; d < 3; ++d)
+  {
+    int const  n_d = topo.nc[d], per = topo.periodic[d];
+    int const  t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    bool const has_l = cc[d] > 0 || per, has_r = cc[d] < n_d - 1 || per;
+    bool const self  = per && n_d == 1;
+    int64_t const fbase = topo.face_base[d] + (int64_t)topo.layers[d] * (cc[t1] + (int64_t)topo.nc[t1] * cc[t2]);
+    int const     fr    = (per && cc[d] + 1 == n_d) ? 0 : cc[d] + 1;
+    int const     tl = has_l ? -1 : topo.bc[2 * d], tr = has_r ? -1 : topo.bc[2 * d + 1];
+    L[d] = line_diagonal(t, n, id[d], h[d], tl, tr, self, (double)tau_face[fbase + cc[d]],
+                         (double)tau_face[fbase + fr]);
+    m[d] = h[d] * t[id[d]];
+  }
+  diag[g] = (Number)(L[0] * m[1] * m[2] + m[0] * L[1] * m[2] + m[0] * m[1] * L[2]);
+}
 }  // namespace
 
 template<typename Number>
@@ -1164,6 +1233,26 @@ hipError_t launch_sipg_cartesian(int degree, ApplyMode mode, SipgArgs<Number> co
   return hipErrorInvalidValue;
 }
 
+template<typename Number>
+hipError_t launch_cartesian_diagonal(Number * diag, double const * diag1d, Number const * tau_face,
+                                     BoxTopology const & topo, double const (&h)[3], int n,
+                                     int64_t n_cells, hipStream_t stream)
+{
+  int64_t const n_dofs = n_cells * n * n * n;
+  if(n_dofs == 0)
+    return hipSuccess;
+  int64_t const blocks = (n_dofs + 255) / 256;
+  hipLaunchKernelGGL((cartesian_diagonal_kernel<Number>), dim3((unsigned)blocks), dim3(256), 0,
+                     stream, diag, diag1d, tau_face, topo, h[0], h[1], h[2], n, n_dofs);
+  return hipGetLastError();
+}
+
+template hipError_t launch_cartesian_diagonal<double>(double *, double const *, double const *,
+                                                      BoxTopology const &, double const (&)[3], int,
+                                                      int64_t, hipStream_t);
+template hipError_t launch_cartesian_diagonal<float>(float *, double const *, float const *,
+                                                     BoxTopology const &, double const (&)[3], int,
+                                                     int64_t, hipStream_t);
 template hipError_t launch_sipg_general<double>(int, ApplyMode, SipgArgs<double> const &, hipStream_t);
 template hipError_t launch_sipg_general<float>(int, ApplyMode, SipgArgs<float> const &, hipStream_t);
 template hipError_t launch_sipg_cartesian<double>(int, ApplyMode, SipgArgs<double> const &, hipStream_t);

@@ -118,6 +118,19 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
     EXADG_THROW(EXADG_ERR_INTERNAL, "1D tables exceed the kernel-argument table");
   tab_.assign(tab.begin(), tab.end());
   tables_.upload(std::vector<Number>(plain.begin(), plain.end()));
+  if(!curved)
+  {
+    // unit-interval 1D diagonals and end derivatives of the closed-form diagonal
+    std::vector<double> dd(4 * (size_t)n);
+    for(int i = 0; i < n; ++i)
+    {
+      dd[i]         = shape_.mass[i * n + i];
+      dd[n + i]     = shape_.stiff[i * n + i];
+      dd[2 * n + i] = shape_.d0[i];
+      dd[3 * n + i] = shape_.d1[i];
+    }
+    diag1d_.upload(dd);
+  }
   for(int d = 0; d < 3; ++d)
   {
     h_[d]     = (Number)grid_->h(d);
@@ -289,7 +302,18 @@ void LaplaceOperator<Number>::calculate_diagonal(Number * diag, hipStream_t s) c
 {
   if(diag == nullptr)
     EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
-  // one launch per local unit vector; each launch writes entry `probe` of every cell
+  if(grid_->affine())
+  {
+    // closed form: one launch (the Kronecker-sum cell block's diagonal)
+    double const     h[3] = {grid_->h(0), grid_->h(1), grid_->h(2)};
+    hipError_t const e    = launch_cartesian_diagonal<Number>(diag, diag1d_.data(), tau_face_.data(),
+                                                           grid_->topology(), h, shape_.n,
+                                                           grid_->n_cells(), s);
+    if(e != hipSuccess)
+      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+    return;
+  }
+  // curved: one launch per local unit vector; each launch writes entry `probe` of every cell
   for(int probe = 0; probe < n3_; ++probe)
     apply(ApplyMode::diagonal, diag, nullptr, probe, s);
 }

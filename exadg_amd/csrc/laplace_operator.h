@@ -176,6 +176,7 @@ private:
   std::vector<Number>                   tab_;  // 1D tables (kernel-argument copy)
   Number                                h_[6] = {1, 1, 1, 1, 1, 1};
   DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
+  DeviceBuffer<double>                  diag1d_;              // affine: M_ii, K_ii, d0, d1 (FP64)
   DeviceBuffer<double>                  div_tables_;          // S, Sd, weights|points (FP64)
   DeviceBuffer<double>                  rhs_tables_;          // S, Sd, d0, d1 (FP64)
   mutable DeviceBuffer<double>          rhs_cell_geo_, rhs_face_geo_;
