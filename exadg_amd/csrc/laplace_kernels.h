@@ -111,6 +111,17 @@ hipError_t launch_cartesian_diagonal(Number * diag, double const * diag1d, Numbe
                                      BoxTopology const & topo, double const (&h)[3], int n,
                                      int64_t n_cells, hipStream_t stream);
 
+// Diagonal of the curved SIPG operator, sum-factorized, one group of n^2 lanes per cell:
+// the cell term sum_q G_ab(q) d_a phi_i d_b phi_i as six three-sweep contractions of the metric
+// with products of the 1D tables, plus the own-side face terms of the face-layer DoFs,
+// c w (tau phi_i^2 - phi_i a.grad phi_i) with c = 1 (interior, neighbour zero), 2 (Dirichlet
+// mirror), 0 (Neumann). tab1d: [SS | DD | SD] (n x n each, [q][i]: S^2, D^2, S D with S the
+// Gauss interpolation and D = Dq S) then [l'(0) of node 0, l'(1) of node n-1]. Meshes with a
+// one-cell periodic direction are not handled (the cell is its own neighbour).
+template<typename Number>
+hipError_t launch_general_diagonal(Number * diag, double const * tab1d, SipgArgs<Number> const & a,
+                                   int degree, hipStream_t stream);
+
 template<typename Number>
 hipError_t launch_sipg_cartesian(int degree, ApplyMode mode, SipgArgs<Number> const & args,
                                  hipStream_t stream);

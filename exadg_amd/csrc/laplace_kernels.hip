@@ -1192,6 +1192,192 @@ __global__ void __launch_bounds__(256)
   }
   diag[g] = (Number)(L[0] * m[1] * m[2] + m[0] * L[1] * m[2] + m[0] * m[1] * L[2]);
 }
+template<int N, typename Number>
+__global__ void __launch_bounds__(256)
+  general_diagonal_kernel(Number * diag, double const * __restrict__ t1d, SipgArgs<Number> a)
+{
+  constexpr int  N2 = N * N, N3 = N * N * N;
+  constexpr int  CPB = cells_per_block(N);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *       smem   = reinterpret_cast<double *>(smem_raw);
+  int const      slot   = threadIdx.x / N2;
+  int const      lt     = threadIdx.x - slot * N2;
+  int const      pos0   = (int)xcd_logical_block(blockIdx.x, gridDim.x) * CPB;
+  bool const     active = slot < CPB && pos0 + slot < (int)a.n_cells;
+  double *       W1     = smem + (active ? slot : 0) * 2 * N3;
+  double *       W2     = W1 + N3;
+  double const * SS = t1d, *DD = t1d + N2, *SD = t1d + 2 * N2;
+  int const      p = lt % N, q = lt / N;
+  int            cc[3] = {0, 0, 0};
+  if(active)
+    traversal_coords(a.topo, a.tile_rows, pos0, slot, cc);
+  int64_t const cell = active ? cell_id(a.topo, cc) : 0;
+  double        acc[N];
+#pragma unroll
+  for(int m = 0; m < N; ++m)
+    acc[m] = 0;
+  // cell term: component c of G (g00, g11, g22, g01, g02, g12) against tables per direction
+  for(int term = 0; term < 6; ++term)
+  {
+    int const      c  = term;
+    double const * T0 = term == 0 ? DD : (term == 3 || term == 4) ? SD : SS;
+    double const * T1 = term == 1 ? DD : (term == 3 || term == 5) ? SD : SS;
+    double const * T2 = term == 2 ? DD : (term == 4 || term == 5) ? SD : SS;
+    double const   f  = term < 3 ? 1.0 : 2.0;
+    if(active)
+    {
+      Number const * G = a.cell_metric + cell * 6 * N3 + (int64_t)c * N3 + N * (p + N * q);
+      double         g[N];
+#pragma unroll
+      for(int q0 = 0; q0 < N; ++q0)
+        g[q0] = (double)G[q0];
+#pragma unroll
+      for(int i0 = 0; i0 < N; ++i0)
+      {
+        double s = 0;
+#pragma unroll
+        for(int q0 = 0; q0 < N; ++q0)
+          s += T0[q0 * N + i0] * g[q0];
+        W1[i0 + N * (p + N * q)] = s;  // (i0, q1 = p, q2 = q)
+      }
+    }
+    __syncthreads();
+    if(active)
+    {
+      double v[N];
+#pragma unroll
+      for(int q1 = 0; q1 < N; ++q1)
+        v[q1] = W1[p + N * (q1 + N * q)];  // (i0 = p, q1, q2 = q)
+#pragma unroll
+      for(int i1 = 0; i1 < N; ++i1)
+      {
+        double s = 0;
+#pragma unroll
+        for(int q1 = 0; q1 < N; ++q1)
+          s += T1[q1 * N + i1] * v[q1];
+        W2[p + N * (i1 + N * q)] = s;
+      }
+    }
+    __syncthreads();
+    if(active)
+    {
+      double v[N];
+#pragma unroll
+      for(int q2 = 0; q2 < N; ++q2)
+        v[q2] = W2[p + N * (q + N * q2)];  // (i0 = p, i1 = q, q2)
+#pragma unroll
+      for(int i2 = 0; i2 < N; ++i2)
+      {
+        double s = 0;
+#pragma unroll
+        for(int q2 = 0; q2 < N; ++q2)
+          s += T2[q2 * N + i2] * v[q2];
+        acc[i2] += f * s;
+      }
+    }
+  }
+  // face terms of the face-layer DoFs
+  if(active)
+  {
+    double const lp[2] = {t1d[3 * N2], t1d[3 * N2 + 1]};  // l'(0) of node 0, l'(1) of node N-1
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+#pragma unroll
+      for(int s = 0; s < 2; ++s)
+      {
+        int const end = s ? N - 1 : 0;
+        // this lane's DoFs on the face: (i_t1, i_t2) and the slot m of its z-line
+        int it1, it2, mfix = -1;
+        if(d == 0)
+        {
+          if(p != end)
+            continue;
+          it1 = q;  // i1, and i2 runs over m
+          it2 = -1;
+        }
+        else if(d == 1)
+        {
+          if(q != end)
+            continue;
+          it1 = p;  // i0, and i2 runs over m
+          it2 = -1;
+        }
+        else
+        {
+          it1  = p;
+          it2  = q;
+          mfix = end;
+        }
+        int64_t const nb = neighbor_id(a.topo, cc, d, s);
+        int const     type = nb >= 0 ? kInterior : a.topo.bc[2 * d + s];
+        double const  cf   = type == kInterior ? 1.0 : type == EXADG_BC_DIRICHLET ? 2.0 : 0.0;
+        if(cf == 0.0)
+          continue;
+        int const      t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+        int64_t const  fid = face_id(a.topo, cc, d, s);
+        Number const * fd  = a.face_data + fid * 7 * N2;
+        double const   tau = (double)a.tau_face[fid];
+        // out(i_t1, i_t2) = sum_{q1,q2} CA SS SS + CB SD SS + CC SS SD
+        double out[N];
+#pragma unroll
+        for(int m = 0; m < N; ++m)
+          out[m] = 0;
+        for(int q2 = 0; q2 < N; ++q2)
+        {
+          double rA = 0, rB = 0, rC = 0;  // sum over q1 with the it1 tables
+#pragma unroll
+          for(int q1 = 0; q1 < N; ++q1)
+          {
+            int const    qq = q1 + N * q2;
+            double const am_d = (double)fd[d * N2 + qq], ap_d = (double)fd[(3 + d) * N2 + qq];
+            double const am_1 = (double)fd[t1 * N2 + qq], ap_1 = (double)fd[(3 + t1) * N2 + qq];
+            double const am_2 = (double)fd[t2 * N2 + qq], ap_2 = (double)fd[(3 + t2) * N2 + qq];
+            double const ad = s ? am_d : -ap_d, a1 = s ? am_1 : -ap_1, a2 = s ? am_2 : -ap_2;
+            double const w  = (double)fd[6 * N2 + qq];
+            double const CA = cf * w * (tau - ad * lp[s]);
+            double const CB = -cf * w * a1;
+            double const CC = -cf * w * a2;
+            rA += CA * SS[q1 * N + it1];
+            rB += CB * SD[q1 * N + it1];
+            rC += CC * SS[q1 * N + it1];
+          }
+          if(mfix >= 0)
+            out[mfix] += (rA + rB) * SS[q2 * N + it2] + rC * SD[q2 * N + it2];
+          else
+#pragma unroll
+            for(int m = 0; m < N; ++m)
+              out[m] += (rA + rB) * SS[q2 * N + m] + rC * SD[q2 * N + m];
+        }
+#pragma unroll
+        for(int m = 0; m < N; ++m)
+          acc[m] += out[m];
+      }
+#pragma unroll
+    for(int m = 0; m < N; ++m)
+      diag[cell * N3 + p + N * q + N2 * m] = (Number)acc[m];
+  }
+}
+
+template<int N, typename Number>
+hipError_t launch_general_diagonal_n(Number * diag, double const * t1d, SipgArgs<Number> const & a,
+                                     hipStream_t stream)
+{
+  constexpr int CPB    = cells_per_block(N);
+  int const     blocks = (int)((a.n_cells + CPB - 1) / CPB);
+  size_t const  lds    = sizeof(double) * CPB * 2 * N * N * N;
+  static std::atomic<bool> attr{false};
+  if(lds > 65536 && !attr.load(std::memory_order_acquire))
+  {
+    hipError_t e = hipFuncSetAttribute((void const *)general_diagonal_kernel<N, Number>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if(e != hipSuccess)
+      return e;
+    attr.store(true, std::memory_order_release);
+  }
+  hipLaunchKernelGGL((general_diagonal_kernel<N, Number>), dim3(blocks), dim3(256), lds, stream,
+                     diag, t1d, a);
+  return hipGetLastError();
+}
 }  // namespace
 
 template<typename Number>
@@ -1247,6 +1433,28 @@ hipError_t launch_cartesian_diagonal(Number * diag, double const * diag1d, Numbe
   return hipGetLastError();
 }
 
+template<typename Number>
+hipError_t launch_general_diagonal(Number * diag, double const * tab1d, SipgArgs<Number> const & a,
+                                   int degree, hipStream_t stream)
+{
+  switch(degree)
+  {
+    case 1: return launch_general_diagonal_n<2, Number>(diag, tab1d, a, stream);
+    case 2: return launch_general_diagonal_n<3, Number>(diag, tab1d, a, stream);
+    case 3: return launch_general_diagonal_n<4, Number>(diag, tab1d, a, stream);
+    case 4: return launch_general_diagonal_n<5, Number>(diag, tab1d, a, stream);
+    case 5: return launch_general_diagonal_n<6, Number>(diag, tab1d, a, stream);
+    case 6: return launch_general_diagonal_n<7, Number>(diag, tab1d, a, stream);
+    case 7: return launch_general_diagonal_n<8, Number>(diag, tab1d, a, stream);
+    case 8: return launch_general_diagonal_n<9, Number>(diag, tab1d, a, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_general_diagonal<double>(double *, double const *,
+                                                    SipgArgs<double> const &, int, hipStream_t);
+template hipError_t launch_general_diagonal<float>(float *, double const *,
+                                                   SipgArgs<float> const &, int, hipStream_t);
 template hipError_t launch_cartesian_diagonal<double>(double *, double const *, double const *,
                                                       BoxTopology const &, double const (&)[3], int,
                                                       int64_t, hipStream_t);

@@ -131,6 +131,25 @@ LaplaceOperator<Number>::LaplaceOperator(std::shared_ptr<StructuredGrid const> g
     }
     diag1d_.upload(dd);
   }
+  else
+  {
+    // tables of the sum-factorized curved diagonal: products of the Gauss values S and the
+    // derivatives D = Dq S of the nodal basis, [q][i] each, then phi_0'(0) and phi_{n-1}'(1)
+    std::vector<double> D((size_t)n * n, 0.0), t((size_t)3 * n * n + 2);
+    for(int q = 0; q < n; ++q)
+      for(int i = 0; i < n; ++i)
+        for(int m = 0; m < n; ++m)
+          D[q * n + i] += shape_.Dq[q * n + m] * shape_.S[m * n + i];
+    for(int k = 0; k < n * n; ++k)
+    {
+      t[k]             = shape_.S[k] * shape_.S[k];
+      t[n * n + k]     = D[k] * D[k];
+      t[2 * n * n + k] = shape_.S[k] * D[k];
+    }
+    t[3 * n * n]     = shape_.d0[0];
+    t[3 * n * n + 1] = shape_.d1[n - 1];
+    diag_tab_.upload(t);
+  }
   for(int d = 0; d < 3; ++d)
   {
     h_[d]     = (Number)grid_->h(d);
@@ -313,7 +332,21 @@ void LaplaceOperator<Number>::calculate_diagonal(Number * diag, hipStream_t s) c
       EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
     return;
   }
-  // curved: one launch per local unit vector; each launch writes entry `probe` of every cell
+  BoxTopology const & t          = grid_->topology();
+  bool                self_periodic = false;
+  for(int d = 0; d < 3; ++d)
+    self_periodic = self_periodic || (t.periodic[d] && t.nc[d] == 1);
+  if(!self_periodic)
+  {
+    // curved: sum-factorized cell and face terms, one launch
+    hipError_t const e = launch_general_diagonal<Number>(diag, diag_tab_.data(),
+                                                         make_args(diag, nullptr, 0), data_.degree, s);
+    if(e != hipSuccess)
+      EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
+    return;
+  }
+  // a cell that is its own neighbour: one launch per local unit vector, each writing entry
+  // `probe` of every cell
   for(int probe = 0; probe < n3_; ++probe)
     apply(ApplyMode::diagonal, diag, nullptr, probe, s);
 }

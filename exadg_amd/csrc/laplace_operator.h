@@ -177,6 +177,7 @@ private:
   Number                                h_[6] = {1, 1, 1, 1, 1, 1};
   DeviceBuffer<Number>                  tables_, cell_metric_, face_data_, tau_face_;
   DeviceBuffer<double>                  diag1d_;              // affine: M_ii, K_ii, d0, d1 (FP64)
+  DeviceBuffer<double>                  diag_tab_;            // curved: S^2, D^2, S D, l'(0), l'(1)
   DeviceBuffer<double>                  div_tables_;          // S, Sd, weights|points (FP64)
   DeviceBuffer<double>                  rhs_tables_;          // S, Sd, d0, d1 (FP64)
   mutable DeviceBuffer<double>          rhs_cell_geo_, rhs_face_geo_;
