@@ -189,12 +189,21 @@ __device__ __forceinline__ void source_line(Number const * __restrict__ src, int
 // normal-derivative vectors a = J^{-1} n of both sides, and the face tests added to the
 // Gauss-level residual before the transposed interpolation back to the nodes.
 
+// FP64 at n = 5, 6: one face direction per round, so that the smaller workspace admits a fourth
+// workgroup per CU
+template<int N, typename Number>
+constexpr bool general_single_rounds()
+{
+  return sizeof(Number) == 8 && (N == 5 || N == 6);
+}
+
 template<int N, typename Number>
 constexpr int general_work_size()
 {
   // cell phase: the x and y gradients (the z one stays in registers); face phases: 8 trace
-  // arrays per direction, two directions per round (x and y, then z)
-  return (2 * N * N * N > 16 * N * N) ? 2 * N * N * N : 16 * N * N;
+  // arrays per direction, two directions per round (x and y, then z) or one
+  constexpr int faces = (general_single_rounds<N, Number>() ? 8 : 16) * N * N;
+  return (2 * N * N * N > faces) ? 2 * N * N * N : faces;
 }
 
 // Face terms from the cell's values at the Gauss points. The own traces at the
@@ -395,13 +404,15 @@ __device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Numb
 }
 
 // Minimum waves per SIMD asked of the compiler for the general kernel, matched to what the
-// LDS allows: FP64 at n = 6 three and at n = 3 four; FP32 at n = 4..6 seven (<= 72 VGPRs, a
-// 20-byte spill at n = 6 that measures no slower than six waves without it), at n = 7 six
+// LDS allows: FP64 at n = 3, 5, 6 four (n = 5, 6 with one face direction per round, whose
+// smaller workspace admits a fourth workgroup per CU; n = 6 spills 28 bytes and still measures
+// 2% faster, n = 5 8% faster); FP32 at n = 4..6 seven (<= 72 VGPRs, a 16-byte spill at n = 6
+// that measures no slower than six waves without it), at n = 7 six
 template<int N, typename Number>
 constexpr int general_min_waves()
 {
   if constexpr(sizeof(Number) == 8)
-    return N == 6 ? 3 : N == 3 ? 4 : 1;
+    return (N == 5 || N == 6) ? 4 : N == 3 ? 4 : 1;
   else
     return (N >= 4 && N <= 6) ? 7 : N == 7 ? 6 : 1;
 }
@@ -503,9 +514,19 @@ __global__ void __launch_bounds__(256)
   // by the transposed derivative sweeps below
   Number fr[3][4] = {};
   {
-    int const xy[2] = {0, 1}, z[1] = {2};
-    gauss_faces<N, Number, mode, 2>(xy, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
-    gauss_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+    if constexpr(general_single_rounds<N, Number>())
+    {
+      int const x[1] = {0}, y[1] = {1}, z[1] = {2};
+      gauss_faces<N, Number, mode, 1>(x, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+      gauss_faces<N, Number, mode, 1>(y, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+      gauss_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+    }
+    else
+    {
+      int const xy[2] = {0, 1}, z[1] = {2};
+      gauss_faces<N, Number, mode, 2>(xy, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+      gauss_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+    }
   }
   Number const * L0 = a.tables + GenTable<N>::L0;
   Number const * L1 = a.tables + GenTable<N>::L1;
