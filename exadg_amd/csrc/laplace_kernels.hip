@@ -405,8 +405,8 @@ __device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Numb
 
 // Minimum waves per SIMD asked of the compiler for the general kernel, matched to what the
 // LDS allows: FP64 at n = 3, 5, 6 four (n = 5, 6 with one face direction per round, whose
-// smaller workspace admits a fourth workgroup per CU; n = 6 spills 28 bytes and still measures
-// 2% faster, n = 5 8% faster); FP32 at n = 4..6 seven (<= 72 VGPRs, a 16-byte spill at n = 6
+// smaller workspace admits a fourth workgroup per CU: n = 5 8% faster, n = 6 4%); FP32 at
+// n = 4..6 seven (<= 72 VGPRs, a 16-byte spill at n = 6
 // that measures no slower than six waves without it), at n = 7 six
 template<int N, typename Number>
 constexpr int general_min_waves()
@@ -445,15 +445,13 @@ __global__ void __launch_bounds__(256)
   // neighbour traces of this lane's line in every direction: value and derivative at the
   // shared face, read up front so that the loads overlap the cell phases
   Number nbw[3][2] = {}, nbdw[3][2] = {};
-  if constexpr(mode != ApplyMode::diagonal)
-    if(active)
-    {
-      Number const * d0 = a.tables + GenTable<N>::D0;
-      Number const * d1 = a.tables + GenTable<N>::D1;
-#pragma unroll
-      for(int d = 0; d < 3; ++d)
+  auto   load_traces = [&](int d) {
+    if constexpr(mode != ApplyMode::diagonal)
+      if(active)
       {
-        int base, stride;
+        Number const * d0 = a.tables + GenTable<N>::D0;
+        Number const * d1 = a.tables + GenTable<N>::D1;
+        int            base, stride;
         line_geometry<N>(d, p, q, base, stride);
 #pragma unroll
         for(int sd = 0; sd < 2; ++sd)
@@ -468,7 +466,15 @@ __global__ void __launch_bounds__(256)
           }
         }
       }
-    }
+  };
+  // FP64 at n = 6 reads the z traces one face round ahead of their use instead of here: the
+  // shorter live range keeps it at 128 VGPRs without a spill (1.8% faster); at n = 5 the exposed
+  // loads cost 7%, so the other instantiations read all traces up front
+  constexpr bool defer_z = general_single_rounds<N, Number>() && N == 6;
+  load_traces(0);
+  load_traces(1);
+  if constexpr(!defer_z)
+    load_traces(2);
 
   // load the cell values
   if(active)
@@ -518,6 +524,8 @@ __global__ void __launch_bounds__(256)
     {
       int const x[1] = {0}, y[1] = {1}, z[1] = {2};
       gauss_faces<N, Number, mode, 1>(x, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
+      if constexpr(defer_z)
+        load_traces(2);
       gauss_faces<N, Number, mode, 1>(y, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
       gauss_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
     }
