@@ -24,6 +24,9 @@ def _cases():
     cases.append(("periodic_xy", 4, (4, 3, 3), 0.0, 1.0, 0.0, 1, 1, (1, 1, 0), (D,) * 4 + (N, N)))
     cases.append(("periodic_curved", 3, (3, 2, 3), 0.0, 1.0, 0.08, 1, 3, (1, 0, 1), (D,) * 6))
     cases.append(("single_periodic", 2, (1, 2, 2), 0.0, 1.0, 0.0, 1, 1, (1, 0, 0), (D,) * 6))
+    # curved with one cell along a periodic direction: the diagonal falls back to probing there
+    cases.append(("single_periodic_curved", 2, (1, 2, 2), 0.0, 1.0, 0.1, 1, 2, (1, 0, 0),
+                  (D,) * 6))
     cases.append(("sine_curv", 3, (4, 4, 4), -1.0, 1.0, 0.15, 2, 3, (0, 0, 0), (D, N, D, D, D, D)))
     return cases
 
@@ -88,8 +91,9 @@ def test_vmult_fp32_matches_oracle(case, torch_cuda, product):
 
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-k{c[1]}" for c in CASES])
 def test_diagonal_fp32_matches_oracle(case, torch_cuda, product):
-    """FP32 diagonal by probing (the Jacobi part of the FP32 smoothers), every mesh case,
-    including a cell that is its own periodic neighbour."""
+    """FP32 diagonal (the Jacobi part of the FP32 smoothers), every mesh case: the closed forms
+    on affine and curved cells, and probing on a curved cell that is its own periodic
+    neighbour."""
     torch = torch_cuda
     op_p, op_o = _build(product, case, product.FP32)
     d_ref = op_o.diagonal()
@@ -99,7 +103,7 @@ def test_diagonal_fp32_matches_oracle(case, torch_cuda, product):
     assert _rel(d.cpu().numpy().astype(np.float64), d_ref) <= 2e-6
 
 
-@pytest.mark.parametrize("case", CASES[::2], ids=[f"{c[0]}-k{c[1]}" for c in CASES[::2]])
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-k{c[1]}" for c in CASES])
 def test_diagonal_matches_oracle(case, torch_cuda, product):
     torch = torch_cuda
     op_p, op_o = _build(product, case, product.FP64)

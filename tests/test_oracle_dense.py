@@ -19,6 +19,8 @@ CASES = [
     ((2, 1, 2), 0.0, 1.0, 0.1, 1, 2, (0, 0, 0), (D,) * 6, 3),
     ((2, 2, 1), 0.0, 1.0, 0.1, 1, 2, (1, 0, 1), (D,) * 6, 2),
     ((3, 1, 1), 0.0, 1.0, 0.0, 1, 1, (1, 1, 0), (D, D, D, D, N, N), 2),
+    # curved, one cell along the periodic x: the cell is its own neighbour across that face
+    ((1, 2, 2), 0.0, 1.0, 0.1, 1, 2, (1, 0, 0), (D,) * 6, 2),
 ]
 
 
