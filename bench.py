@@ -86,19 +86,20 @@ def curved_bytes(k, n_cells_1d, s=8):
     n = k + 1
     nc = n_cells_1d ** 3
     n_faces = 3 * n_cells_1d ** 2 * (n_cells_1d + 1)
-    return 2 * s * nc * n ** 3 + 6 * s * n ** 3 * nc + 7 * s * n ** 2 * n_faces + 8 * n_faces
+    return 2 * s * nc * n ** 3 + 6 * s * n ** 3 * nc + 7 * s * n ** 2 * n_faces + s * n_faces
 
 
-def run_sweep(ex, torch, args):
+def run_sweep(ex, torch, args, fp32=False):
     """Curved-mesh operator sweep: [0,1]^3 deformed by x_d += 0.1 prod sin(pi x_e), MappingQ(k),
-    k = 2, 3, 5, 7 at ~23.9M DoF each, FP64 vmult, seeded uniform[-1,1] input, median of
-    HIP-event timings."""
+    k = 2, 3, 5, 7 at ~23.9M DoF each, FP64 (or FP32) vmult, seeded uniform[-1,1] input, median
+    of HIP-event timings."""
     res = []
+    dtype, s = (torch.float32, 4) if fp32 else (torch.float64, 8)
     for k, m in ((2, 96), (3, 72), (5, 48), (7, 36)):
         op = ex.LaplaceOperator(ex.mesh_desc(m, deformation=0.1, frequency=1, mapping_degree=k),
-                                ex.bc_desc(), k, ex.FP64)
+                                ex.bc_desc(), k, ex.FP32 if fp32 else ex.FP64)
         gen = torch.Generator(device="cuda").manual_seed(k)
-        x = torch.rand(op.n, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1
+        x = torch.rand(op.n, dtype=torch.float64, device="cuda", generator=gen).to(dtype) * 2 - 1
         y = torch.empty_like(x)
         for _ in range(3):
             op.vmult(y, x)
@@ -111,7 +112,7 @@ def run_sweep(ex, torch, args):
             b.record(stream)
         torch.cuda.synchronize()
         t = statistics.median(a.elapsed_time(b) for a, b in ev) * 1e-3
-        nb = curved_bytes(k, m)
+        nb = curved_bytes(k, m, s)
         res.append({"degree": k, "cells": f"{m}^3", "n_dofs": op.n,
                     "gdofs_per_s": round(op.n / t / 1e9, 2), "ms": round(t * 1e3, 4),
                     "algorithmic_GBps": round(nb / t / 1e9, 1),
@@ -271,6 +272,8 @@ def main():
                     help="also time the solve on the Cartesian k=3 8^3 box (launch-bound)")
     ap.add_argument("--sweep", action="store_true",
                     help="also time the curved-mesh vmult at degrees 2, 3, 5, 7 (~24M DoF each)")
+    ap.add_argument("--sweep-fp32", action="store_true",
+                    help="with --sweep: the same sweep in FP32 as well (curved_sweep_fp32)")
     ap.add_argument("--guess", action="store_true",
                     help="also run the learned-initial-guess pressure sequence (32^3, k=4)")
     ap.add_argument("--guess-cells", type=int, default=32)
@@ -391,6 +394,8 @@ def main():
         x = y = None
         torch.cuda.empty_cache()
         out["curved_sweep"] = run_sweep(exadg_amd, torch, args)
+        if args.sweep_fp32:
+            out["curved_sweep_fp32"] = run_sweep(exadg_amd, torch, args, fp32=True)
     if args.guess:
         x = y = None
         torch.cuda.empty_cache()

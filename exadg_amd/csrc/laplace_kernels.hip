@@ -197,6 +197,18 @@ constexpr bool general_single_rounds()
   return sizeof(Number) == 8 && (N == 5 || N == 6);
 }
 
+// The z derivative of the Gauss values taken after the face phase, so that it is not held
+// across the face rounds: it removes the spills of FP32 n = 6 and FP64 n = 6 and lowers the
+// register counts (FP64 n = 9 173 -> 132 VGPRs, three waves per SIMD instead of two), 4% for
+// the V-cycles of the curved k = 5 hierarchy and 6% for the FP64 k = 5 vmult. FP64 at n = 8
+// stays at three waves per SIMD either way and only pays the extra LDS reads (7%), except the
+// Chebyshev step, which it brings from two waves to three (k = 7 V-cycle 8% faster).
+template<int N, typename Number, ApplyMode mode>
+constexpr bool general_late_gz()
+{
+  return !(sizeof(Number) == 8 && N == 8 && mode != ApplyMode::chebyshev);
+}
+
 template<int N, typename Number>
 constexpr int general_work_size()
 {
@@ -406,8 +418,7 @@ __device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Numb
 // Minimum waves per SIMD asked of the compiler for the general kernel, matched to what the
 // LDS allows: FP64 at n = 3, 5, 6 four (n = 5, 6 with one face direction per round, whose
 // smaller workspace admits a fourth workgroup per CU: n = 5 8% faster, n = 6 4%); FP32 at
-// n = 4..6 seven (<= 72 VGPRs, a 16-byte spill at n = 6
-// that measures no slower than six waves without it), at n = 7 six
+// n = 4..6 seven (<= 72 VGPRs), at n = 7 six
 template<int N, typename Number>
 constexpr int general_min_waves()
 {
@@ -467,14 +478,9 @@ __global__ void __launch_bounds__(256)
         }
       }
   };
-  // FP64 at n = 6 reads the z traces one face round ahead of their use instead of here: the
-  // shorter live range keeps it at 128 VGPRs without a spill (1.8% faster); at n = 5 the exposed
-  // loads cost 7%, so the other instantiations read all traces up front
-  constexpr bool defer_z = general_single_rounds<N, Number>() && N == 6;
   load_traces(0);
   load_traces(1);
-  if constexpr(!defer_z)
-    load_traces(2);
+  load_traces(2);
 
   // load the cell values
   if(active)
@@ -505,15 +511,19 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   // z sweep, then the z derivative at the Gauss points: this lane's z-line (points lt + m N2),
-  // kept in registers -- the same lane applies the metric at those points below
-  Number gz_line[N];
+  // kept in registers -- the same lane applies the metric at those points below. With
+  // late_gz the derivative is taken after the face phase from the z-line in A instead, so that
+  // it is not live across the face rounds (the same values)
+  constexpr bool late_gz = general_late_gz<N, Number, mode>();
+  Number         gz_line[N];
   if(active)
   {
     Number v[N], o[N];
     load_line<N>(A + p + N * q, N2, v);
     gen_apply<N, false>(SE, v, o);
     store_line<N>(A + p + N * q, N2, o);
-    gen_apply<N, true>(DE, o, gz_line);
+    if constexpr(!late_gz)
+      gen_apply<N, true>(DE, o, gz_line);
   }
   __syncthreads();
   // the face terms from the Gauss values in A: four result scalars per direction per lane, added
@@ -524,8 +534,6 @@ __global__ void __launch_bounds__(256)
     {
       int const x[1] = {0}, y[1] = {1}, z[1] = {2};
       gauss_faces<N, Number, mode, 1>(x, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
-      if constexpr(defer_z)
-        load_traces(2);
       gauss_faces<N, Number, mode, 1>(y, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
       gauss_faces<N, Number, mode, 1>(z, a, cell, cc, active, lt, A, W, nbw, nbdw, fr);
     }
@@ -555,6 +563,11 @@ __global__ void __launch_bounds__(256)
     load_line<N>(A + p + N2 * q, N, v);
     gen_apply<N, true>(DE, v, g);
     store_line<N>(Gy + p + N2 * q, N, g);
+    if constexpr(late_gz)
+    {
+      load_line<N>(A + p + N * q, N2, v);
+      gen_apply<N, true>(DE, v, gz_line);
+    }
   }
   __syncthreads();
   // metric: (fx,fy,fz) = G (gx,gy,gz), G symmetric per quadrature point
