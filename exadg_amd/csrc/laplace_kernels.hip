@@ -417,15 +417,15 @@ __device__ __forceinline__ void gauss_faces(int const (&dirs)[ND], SipgArgs<Numb
 
 // Minimum waves per SIMD asked of the compiler for the general kernel, matched to what the
 // LDS allows: FP64 at n = 3, 5, 6 four (n = 5, 6 with one face direction per round, whose
-// smaller workspace admits a fourth workgroup per CU: n = 5 8% faster, n = 6 4%); FP32 at
-// n = 4..6 seven (<= 72 VGPRs), at n = 7 six
+// smaller workspace admits a fourth workgroup per CU: n = 5 8% faster, n = 6 4%); FP32 at n = 3
+// eight (<= 64 VGPRs, 19 KB of LDS per workgroup), at n = 4..6 seven (<= 72 VGPRs), at n = 7 six
 template<int N, typename Number>
 constexpr int general_min_waves()
 {
   if constexpr(sizeof(Number) == 8)
     return (N == 5 || N == 6) ? 4 : N == 3 ? 4 : 1;
   else
-    return (N >= 4 && N <= 6) ? 7 : N == 7 ? 6 : 1;
+    return N == 3 ? 8 : (N >= 4 && N <= 6) ? 7 : N == 7 ? 6 : 1;
 }
 
 template<int N, typename Number, ApplyMode mode>
