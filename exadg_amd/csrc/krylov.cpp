@@ -30,12 +30,27 @@ SolveInfo cg_solve_in(Poisson::LaplaceOperator<double> const & A, Preconditioner
   if(!P)
     z = r;
   SolveInfo info;
+  // A FP32 preconditioner is applied in its own precision: the update kernel also writes the
+  // residual rounded to FP32 into the preconditioner's input, and the dot product and the
+  // direction update read its FP32 result, widened exactly. The iterates are the same as with
+  // the FP64 vmult of the preconditioner, without its two conversions.
+  float * const r_narrow = P ? P->narrow_input() : nullptr;
+  float const * z_narrow = nullptr;
 
   // r = b - A x, z = P r
   A.residual(r, x, b, s);
-  if(P)
-    P->vmult(z, r, s);
-  vec::norm2_and_dot(red, 1, 2, r, z, n, s);
+  if(r_narrow)
+  {
+    vec::convert<float, double>(r_narrow, r, n, s);
+    z_narrow = P->vmult_narrow(s);
+    vec::norm2_and_dot(red, 1, 2, r, z_narrow, n, s);
+  }
+  else
+  {
+    if(P)
+      P->vmult(z, r, s);
+    vec::norm2_and_dot(red, 1, 2, r, z, n, s);
+  }
   vec::dot(red, 4, b, b, n, s);
   double const * h = red.fetch(s, 5);
   info.r0          = std::sqrt(h[1]);
@@ -54,16 +69,28 @@ SolveInfo cg_solve_in(Poisson::LaplaceOperator<double> const & A, Preconditioner
     info.converged = true;
     return info;
   }
-  vec::copy(p, z, n, s);
+  if(z_narrow)
+    vec::convert<double, float>(p, z_narrow, n, s);
+  else
+    vec::copy(p, z, n, s);
   for(int it = 1; it <= data.max_iter; ++it)
   {
     int const old_slot = 2 + (it - 1) % 2, new_slot = 2 + it % 2;
     A.vmult(v, p, s);
     vec::dot(red, 0, p, v, n, s);
-    vec::cg_update(x, r, p, v, red.result() + old_slot, red.result(), n, s);
-    if(P)
-      P->vmult(z, r, s);
-    vec::norm2_and_dot(red, 1, new_slot, r, z, n, s);
+    if(r_narrow)
+    {
+      vec::cg_update_narrow(x, r, r_narrow, p, v, red.result() + old_slot, red.result(), n, s);
+      z_narrow = P->vmult_narrow(s);
+      vec::norm2_and_dot(red, 1, new_slot, r, z_narrow, n, s);
+    }
+    else
+    {
+      vec::cg_update(x, r, p, v, red.result() + old_slot, red.result(), n, s);
+      if(P)
+        P->vmult(z, r, s);
+      vec::norm2_and_dot(red, 1, new_slot, r, z, n, s);
+    }
     h               = red.fetch(s, 4);
     double const rn = std::sqrt(h[1]);
     info.iterations = it;
@@ -78,7 +105,10 @@ SolveInfo cg_solve_in(Poisson::LaplaceOperator<double> const & A, Preconditioner
       info.converged = true;
       break;
     }
-    vec::cg_direction(p, z, red.result() + new_slot, red.result() + old_slot, n, s);
+    if(z_narrow)
+      vec::cg_direction(p, z_narrow, red.result() + new_slot, red.result() + old_slot, n, s);
+    else
+      vec::cg_direction(p, z, red.result() + new_slot, red.result() + old_slot, n, s);
   }
   if(!info.not_finite)
   {

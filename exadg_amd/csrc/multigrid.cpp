@@ -1209,17 +1209,23 @@ void MultigridPreconditioner<Number>::vmult_eager(double * dst, double const * s
 {
   int const     top = n_levels() - 1;
   int64_t const n   = levels_[top].op->n();
-  if constexpr(std::is_same<Number, double>::value)
-    vec::copy(levels_[top].defect.data(), src, n, s);
-  else
-    vec::convert<Number, double>(levels_[top].defect.data(), src, n, s);
+  if(src)
+  {
+    if constexpr(std::is_same<Number, double>::value)
+      vec::copy(levels_[top].defect.data(), src, n, s);
+    else
+      vec::convert<Number, double>(levels_[top].defect.data(), src, n, s);
+  }
   for(int l = 0; l < top; ++l)
     vec::set_zero(levels_[l].defect.data(), levels_[l].op->n(), s);
   v_cycle(top, s);
-  if constexpr(std::is_same<Number, double>::value)
-    vec::copy(dst, levels_[top].sol, n, s);
-  else
-    vec::convert<double, Number>(dst, levels_[top].sol, n, s);
+  if(dst)
+  {
+    if constexpr(std::is_same<Number, double>::value)
+      vec::copy(dst, levels_[top].sol, n, s);
+    else
+      vec::convert<double, Number>(dst, levels_[top].sol, n, s);
+  }
 }
 
 template<typename Number>
@@ -1242,16 +1248,44 @@ bool MultigridPreconditioner<Number>::smoother_vmult(double * dst, double const 
 template<typename Number>
 void MultigridPreconditioner<Number>::vmult(double * dst, double const * src, hipStream_t s)
 {
+  if(dst == nullptr || src == nullptr)
+    EXADG_THROW(EXADG_ERR_ARGUMENT, "null vector");
+  run(dst, src, s);
+}
+
+template<typename Number>
+float * MultigridPreconditioner<Number>::narrow_input()
+{
+  if constexpr(std::is_same<Number, float>::value)
+    return levels_.back().defect.data();
+  else
+    return nullptr;
+}
+
+template<typename Number>
+float const * MultigridPreconditioner<Number>::vmult_narrow(hipStream_t s)
+{
+  if constexpr(std::is_same<Number, float>::value)
+    return run(nullptr, nullptr, s);
+  else
+    return nullptr;
+}
+
+template<typename Number>
+Number const * MultigridPreconditioner<Number>::run(double * dst, double const * src, hipStream_t s)
+{
   // The first call runs eagerly (kernel attributes get set outside any capture). Later calls
   // replay a graph of the whole V-cycle: its ~10^2 small launches on the coarse levels cost
   // more in launch latency than in execution. Every buffer a cycle reads is written earlier in
-  // the same cycle, so each replay computes the same function of src. The capture records
-  // only kernels and device copies (no memset nodes) on a private stream; the replay runs on s.
+  // the same cycle (or, with src = NULL, is the finest defect), so each replay computes the
+  // same function of its input. The capture records only kernels and device copies (no memset
+  // nodes) on a private stream; the replay runs on s. The smoothers swap the solution vectors
+  // on the host, so each graph keeps the finest solution vector it ends in.
   if(!uses_graph() || eager_calls_ == 0)
   {
     vmult_eager(dst, src, s);
     ++eager_calls_;
-    return;
+    return levels_.back().sol;
   }
   Graph const * g = nullptr;
   for(Graph const & c : graphs_)
@@ -1282,10 +1316,11 @@ void MultigridPreconditioner<Number>::vmult(double * dst, double const * src, hi
     (void)hipGraphDestroy(graph);
     if(e != hipSuccess)
       EXADG_THROW(EXADG_ERR_HIP, hipGetErrorString(e));
-    graphs_.push_back({dst, src, exec});
+    graphs_.push_back({dst, src, exec, levels_.back().sol});
     g = &graphs_.back();
   }
   HIP_CHECK(hipGraphLaunch(g->exec, s));
+  return g->out;
 }
 
 std::unique_ptr<PreconditionerBase> create_multigrid(std::shared_ptr<StructuredGrid const> grid,

@@ -38,6 +38,13 @@ public:
   }
   // true if vmult may allocate device memory or HIP objects (graph capture on first use)
   virtual bool vmult_allocates() const { return false; }
+  // For an outer FP64 Krylov solver around a FP32 preconditioner: narrow_input() is the FP32
+  // vector that vmult_narrow reads, and vmult_narrow applies the preconditioner to that vector
+  // as it stands and returns its FP32 result (valid until the next call). vmult(dst, src) equals
+  // rounding src into narrow_input(), vmult_narrow and widening the result into dst. nullptr
+  // when the preconditioner has no FP32 form.
+  virtual float * narrow_input() { return nullptr; }
+  virtual float const * vmult_narrow(hipStream_t /*s*/) { return nullptr; }
 };
 
 // BlockJacobiPreconditioner (block_jacobi_preconditioner.h) on the FP64 operator: the exact
@@ -296,6 +303,8 @@ public:
   void update(hipStream_t s) override;
   bool smoother_vmult(double * dst, double const * src, hipStream_t s) override;
   bool vmult_allocates() const override { return uses_graph(); }
+  float *       narrow_input() override;
+  float const * vmult_narrow(hipStream_t s) override;
 
   int n_levels() const { return (int)levels_.size(); }
   MGLevelInfo level_info(int l) const { return {levels_[l].h, levels_[l].degree, levels_[l].dg}; }
@@ -332,14 +341,18 @@ private:
   };
   void v_cycle(int l, hipStream_t s);
   void coarse_solve(hipStream_t s);
-  void vmult_eager(double * dst, double const * src, hipStream_t s);
-  void clear_graphs();
+  // one V-cycle from src (NULL: the finest defect as it stands) into dst (NULL: none); returns
+  // the finest level's solution vector of this cycle
+  Number const * run(double * dst, double const * src, hipStream_t s);
+  void           vmult_eager(double * dst, double const * src, hipStream_t s);
+  void           clear_graphs();
 
   struct Graph
   {
     double *        dst;
     double const *  src;
     hipGraphExec_t  exec;
+    Number const *  out;  // the finest solution vector the captured cycle ends in
   };
   std::vector<Graph> graphs_;
   hipStream_t        capture_stream_ = nullptr;

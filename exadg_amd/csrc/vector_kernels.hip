@@ -66,10 +66,11 @@ __device__ __forceinline__ double block_sum(double v)
 // One-launch deterministic reduction: every block writes its partial sum(s), and the block
 // that finishes last (ticket from a device-scope counter) adds the partials in block order,
 // writes the result slot(s) and resets the counter to zero for the next reduction.
-// K = 0: out0 = sum a*b; K = 1: out0 = sum a*b and out1 = sum a*a.
-template<typename Number, int K>
+// K = 0: out0 = sum a*b; K = 1: out0 = sum a*b and out1 = sum a*a. b may be narrower than a
+// (the FP32 result of a mixed-precision preconditioner): its entries widen exactly to FP64.
+template<typename Number, int K, typename NumberB = Number>
 __global__ void __launch_bounds__(kBlock)
-  reduce_fused(Number const * a, Number const * b, int64_t n, double * partials,
+  reduce_fused(Number const * a, NumberB const * b, int64_t n, double * partials,
                unsigned int * counter, double * out0, double * out1)
 {
   double        s0 = 0, s1 = 0;
@@ -190,15 +191,32 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template<typename Number>
+// the CG update that also writes the new residual rounded to the preconditioner's precision
+template<typename Number, typename Narrow>
 __global__ void __launch_bounds__(kBlock)
-  cg_direction_kernel(Number * p, Number const * z, double const * num, double const * den,
+  cg_update_narrow_kernel(Number * x, Number * r, Narrow * r_narrow, Number const * p,
+                          Number const * v, double const * num, double const * den, int64_t n)
+{
+  Number const  alpha  = (Number)(*num / *den);
+  int64_t const stride = (int64_t)gridDim.x * kBlock;
+  for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+  {
+    x[i] += alpha * p[i];
+    Number const ri = r[i] - alpha * v[i];
+    r[i]            = ri;
+    r_narrow[i]     = (Narrow)ri;
+  }
+}
+
+template<typename Number, typename NumberZ = Number>
+__global__ void __launch_bounds__(kBlock)
+  cg_direction_kernel(Number * p, NumberZ const * z, double const * num, double const * den,
                       int64_t n)
 {
   Number const  beta   = (Number)(*num / *den);
   int64_t const stride = (int64_t)gridDim.x * kBlock;
   for(int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-    p[i] = z[i] + beta * p[i];
+    p[i] = (Number)z[i] + beta * p[i];
 }
 
 template<typename Number>
@@ -341,12 +359,12 @@ void dot(Reducer & red, int slot, Number const * a, Number const * b, int64_t n,
   HIP_CHECK(hipGetLastError());
 }
 
-template<typename Number>
-void norm2_and_dot(Reducer & red, int slot_aa, int slot_ab, Number const * a, Number const * b,
+template<typename Number, typename NumberB>
+void norm2_and_dot(Reducer & red, int slot_aa, int slot_ab, Number const * a, NumberB const * b,
                    int64_t n, hipStream_t s)
 {
   int const g = grid_for(n, Reducer::kMaxPartials);
-  hipLaunchKernelGGL((reduce_fused<Number, 1>), dim3(g), dim3(kBlock), 0, s, a, b, n,
+  hipLaunchKernelGGL((reduce_fused<Number, 1, NumberB>), dim3(g), dim3(kBlock), 0, s, a, b, n,
                      red.partials(), red.counter(), red.result() + slot_ab, red.result() + slot_aa);
   HIP_CHECK(hipGetLastError());
 }
@@ -367,12 +385,21 @@ void cg_update(Number * x, Number * r, Number const * p, Number const * v, doubl
   HIP_CHECK(hipGetLastError());
 }
 
-template<typename Number>
-void cg_direction(Number * p, Number const * z, double const * num, double const * den, int64_t n,
+template<typename Number, typename NumberZ>
+void cg_direction(Number * p, NumberZ const * z, double const * num, double const * den, int64_t n,
                   hipStream_t s)
 {
-  hipLaunchKernelGGL((cg_direction_kernel<Number>), dim3(grid_for(n)), dim3(kBlock), 0, s, p, z,
-                     num, den, n);
+  hipLaunchKernelGGL((cg_direction_kernel<Number, NumberZ>), dim3(grid_for(n)), dim3(kBlock), 0, s,
+                     p, z, num, den, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+template<typename Number, typename Narrow>
+void cg_update_narrow(Number * x, Number * r, Narrow * r_narrow, Number const * p, Number const * v,
+                      double const * num, double const * den, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL((cg_update_narrow_kernel<Number, Narrow>), dim3(grid_for(n)), dim3(kBlock), 0,
+                     s, x, r, r_narrow, p, v, num, den, n);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -426,6 +453,13 @@ EXADG_INST(float)
 #undef EXADG_INST
 template void convert<float, double>(float *, double const *, int64_t, hipStream_t);
 template void convert<double, float>(double *, float const *, int64_t, hipStream_t);
+template void norm2_and_dot<double, float>(Reducer &, int, int, double const *, float const *,
+                                           int64_t, hipStream_t);
+template void cg_direction<double, float>(double *, float const *, double const *, double const *,
+                                          int64_t, hipStream_t);
+template void cg_update_narrow<double, float>(double *, double *, float *, double const *,
+                                              double const *, double const *, double const *,
+                                              int64_t, hipStream_t);
 }  // namespace vec
 
 namespace Poisson

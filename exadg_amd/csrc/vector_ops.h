@@ -82,9 +82,10 @@ void convert(To * dst, From const * src, int64_t n, hipStream_t s);
 // result[slot] = sum a_i b_i (accumulated in FP64, deterministic two-stage reduction)
 template<typename Number>
 void dot(Reducer & red, int slot, Number const * a, Number const * b, int64_t n, hipStream_t s);
-// result[slot_aa] = sum a_i^2 and result[slot_ab] = sum a_i b_i in one pass
-template<typename Number>
-void norm2_and_dot(Reducer & red, int slot_aa, int slot_ab, Number const * a, Number const * b,
+// result[slot_aa] = sum a_i^2 and result[slot_ab] = sum a_i b_i in one pass (b may be FP32
+// with a FP64: its entries widen exactly)
+template<typename Number, typename NumberB>
+void norm2_and_dot(Reducer & red, int slot_aa, int slot_ab, Number const * a, NumberB const * b,
                    int64_t n, hipStream_t s);
 // y = a x + b y
 template<typename Number>
@@ -93,10 +94,14 @@ void axpby(Number * y, Number a, Number const * x, Number b, int64_t n, hipStrea
 template<typename Number>
 void cg_update(Number * x, Number * r, Number const * p, Number const * v, double const * num,
                double const * den, int64_t n, hipStream_t s);
-// p = z + (num/den) p
-template<typename Number>
-void cg_direction(Number * p, Number const * z, double const * num, double const * den, int64_t n,
+// p = z + (num/den) p (z may be FP32 with p FP64)
+template<typename Number, typename NumberZ>
+void cg_direction(Number * p, NumberZ const * z, double const * num, double const * den, int64_t n,
                   hipStream_t s);
+// cg_update that also writes r_narrow = r rounded to Narrow (the input of a FP32 preconditioner)
+template<typename Number, typename Narrow>
+void cg_update_narrow(Number * x, Number * r, Narrow * r_narrow, Number const * p, Number const * v,
+                      double const * num, double const * den, int64_t n, hipStream_t s);
 // z = d .* r
 template<typename Number>
 void scale(Number * z, Number const * d, Number const * r, int64_t n, hipStream_t s);
