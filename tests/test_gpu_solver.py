@@ -134,6 +134,18 @@ def test_vcycle_graph_replays_match_eager(case, precision, torch_cuda, product):
     ig = product.cg_solve(op, b, xg, pg, rel_tol=1e-10, rel_mode=1)
     ie = product.cg_solve(op, b, xe, pe, rel_tol=1e-10, rel_mode=1)
     assert ig["iterations"] == ie["iterations"] and ig["converged"]
+    xscale = float(xe.abs().max())
+    assert float((xg - xe).abs().max()) <= 1e-12 * xscale
+    # another (dst, src) pair captured after the solve's own graph, then a second solve that
+    # replays the solve's graph: each graph returns the solution vector it was captured with
+    z3 = torch.full_like(r, float("nan"))
+    pg.vmult(z3, r)
+    torch.cuda.synchronize()
+    assert float((z3 - ze).abs().max()) <= 1e-14 * scale
+    xg2 = torch.zeros_like(b)
+    ig2 = product.cg_solve(op, b, xg2, pg, rel_tol=1e-10, rel_mode=1)
+    assert ig2["iterations"] == ig["iterations"]
+    assert float((xg2 - xg).abs().max()) <= 1e-14 * xscale
 
 
 def _sine(product, k, curvilinear):
