@@ -164,6 +164,18 @@ __device__ __forceinline__ void line_geometry(int d, int p, int q, int & base, i
   }
 }
 
+// A store of an output that no kernel of the same apply reads back: marked non-temporal, so
+// that it does not displace from the caches the neighbour cells that other workgroups are
+// still reading. Used for the Cartesian vmult (FP64 k = 4: 7% faster on 64^3 cells, 3.5% on
+// 32^3; FP32 unchanged). The Chebyshev and residual outputs are read by the next kernel of a
+// V-cycle and stay cached (non-temporal there: V-cycle 0.8% slower), and the general kernel
+// keeps plain stores (0-3% slower with these).
+template<typename Number>
+__device__ __forceinline__ void store_stream(Number * p, Number v)
+{
+  __builtin_nontemporal_store(v, p);
+}
+
 // the cell's unit vector in diagonal (probing) mode, else the src line
 template<int N, typename Number, ApplyMode mode>
 __device__ __forceinline__ void source_line(Number const * __restrict__ src, int64_t cell, int base,
@@ -1085,7 +1097,7 @@ __global__ void __launch_bounds__(256) sipg_cartesian_kernel(SipgArgs<Number> a)
       Number const ax = outp[k] + v[k];
       size_t const g  = g0 + k * N2;
       if constexpr(mode == ApplyMode::vmult)
-        a.dst[g] = ax;
+        store_stream(&a.dst[g], ax);
       else if constexpr(mode == ApplyMode::vmult_add)
         a.dst[g] += ax;
       else if constexpr(mode == ApplyMode::residual)
