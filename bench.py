@@ -370,7 +370,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "limiter": "latency of the per-workgroup load -> LDS phases -> store chain, "
-                                "not HBM bytes (traffic ~1.1x algorithmic) nor VALU/LDS "
+                                "not HBM bytes (traffic ~1.2x algorithmic) nor VALU/LDS "
                                 "throughput"},
         "stream": stream_bw,
     }
